@@ -1,0 +1,66 @@
+"""Build recipe of the native parts, in-tree.
+
+    python -m aphid_git_amd.build
+
+compiles libaphid_gpu.so (hipcc, gfx950) from aphid_git_amd/csrc and the
+Fortran driver (amdflang) from aphid_git_amd/fortran.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+FSRC = os.path.join(HERE, "fortran")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("APHID_FC", "/opt/rocm/bin/amdflang")
+ARCH = os.environ.get("APHID_ARCH", "gfx950")
+
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+             # packed-FP32 instructions are kept off for all device code
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_hip(verbose_resources: bool = False) -> str:
+    out = os.path.join(HERE, "libaphid_gpu.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "aphid_gpu.hip")]
+    if verbose_resources:
+        cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran() -> str:
+    """Fortran driver: iso_c_binding module + the aphid_gpu_serial program."""
+    if not os.path.exists(FSRC) or not os.path.exists(FC):
+        return ""
+    bdir = os.path.join(HERE, "fortran", "_mod")
+    os.makedirs(bdir, exist_ok=True)
+    srcs = [os.path.join(FSRC, f) for f in ("aphid_gpu_mod.f90", "aphid_driver.f90")]
+    if not all(os.path.exists(s) for s in srcs):
+        return ""
+    exe = os.path.join(HERE, "aphid_gpu_serial")
+    _run([FC, "-O2", "-J" + bdir, *srcs, "-o", exe, "-L" + HERE, "-laphid_gpu",
+          "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    return exe
+
+
+def main():
+    build_hip(verbose_resources="-v" in sys.argv)
+    build_fortran()
+
+
+if __name__ == "__main__":
+    main()

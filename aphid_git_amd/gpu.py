@@ -1,0 +1,266 @@
+"""Python binding of libaphid_gpu.so (include/aphid_gpu.h) via ctypes.
+
+This mirrors the reference's interface for the hot path: the operator
+apply (inprod_x/y/z/p, src/aphid.f90:803-1282), the setup it depends on
+(gen_k2_*grid, gen_diag, gen_inprod_*_coefs, read_mesh's sigma models,
+gen_rhs) and the QMR solve (qmr, src/aphid_qmr.f90:5-434), with the
+reference's array shapes and in-place solution convention.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+visible, constructing a :class:`Solver` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaphid_gpu.so")
+_lib = None
+
+F_X0 = 1
+F_NO_TRUE_RES = 2
+F_MIXED = 4
+STATUS = {0: "converged", 1: "max_it", 2: "breakdown"}
+
+
+class AphidError(RuntimeError):
+    pass
+
+
+class Report(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("rnorm", C.c_double),
+                ("rhsnorm", C.c_double), ("true_rel_res", C.c_double),
+                ("seconds_qmr", C.c_double), ("seconds_total", C.c_double),
+                ("hist", C.POINTER(C.c_double)), ("hist_cap", C.c_int),
+                ("hist_len", C.c_int), ("outer_iters", C.c_int), ("reserved", C.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AphidError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.aphid_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def comp_shapes(nx, ny, nz) -> List[Tuple[int, int, int]]:
+    return [(nx - 1, ny, nz), (nx, ny - 1, nz), (nx, ny, nz - 1), (nx, ny, nz)]
+
+
+def _c128(a, shape):
+    a = np.asfortranarray(a, dtype=np.complex128)
+    if a.shape != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise AphidError(f"{what}: {lib().aphid_last_error().decode()} (code {rc})")
+    return rc
+
+
+@dataclasses.dataclass
+class SolveResult:
+    x: List[np.ndarray]
+    iters: int
+    status: int
+    rnorm: float
+    rhsnorm: float
+    true_rel_res: float
+    seconds_qmr: float
+    seconds_total: float
+    outer_iters: int
+    hist: np.ndarray
+
+    @property
+    def converged(self) -> bool:
+        return self.status == 0
+
+
+class Solver:
+    """One device context: mesh, operator data and Krylov workspace."""
+
+    def __init__(self, device: int = -1):
+        self._ctx = C.c_void_p()
+        _check(lib().aphid_ctx_create(C.byref(self._ctx), C.c_int(device)), "aphid_ctx_create")
+        self.shape: Optional[Tuple[int, int, int]] = None
+
+    def close(self):
+        if self._ctx:
+            lib().aphid_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- setup -------------------------------------------------------------
+    def set_mesh(self, xn, yn, zn):
+        xn, yn, zn = (np.ascontiguousarray(a, np.float64) for a in (xn, yn, zn))
+        _check(lib().aphid_set_mesh(self._ctx, len(xn), len(yn), len(zn), _p(xn), _p(yn), _p(zn)),
+               "aphid_set_mesh")
+        self.shape = (len(xn), len(yn), len(zn))
+
+    def set_cells(self, model: str, omega: float, cells: dict):
+        nx, ny, nz = self.shape
+        cs = (nx - 1, ny - 1, nz - 1)
+
+        def f(name):
+            a = cells.get(name)
+            if a is None:
+                return None
+            a = np.asfortranarray(a, dtype=np.float32)
+            if a.shape != cs:
+                raise ValueError(f"cell block {name}: expected {cs}, got {a.shape}")
+            return a
+
+        blocks = {"R": ("sigma", None, None, None), "C": ("sigma", "eps_rel", None, None),
+                  "D": ("sigma", "eta", "tau", "cee"), "I": (None, "eps_rel", None, None)}[model]
+        arrs = [f(n) if n else None for n in blocks]
+        _check(lib().aphid_set_cells(self._ctx, C.c_char(model.encode()), C.c_double(omega),
+                                     *[_p(a) for a in arrs]), "aphid_set_cells")
+
+    def set_sigma(self, sig, omega: float):
+        nx, ny, nz = self.shape
+        sig = _c128(sig, (nx - 1, ny - 1, nz - 1))
+        _check(lib().aphid_set_sigma(self._ctx, _p(sig), C.c_double(omega)), "aphid_set_sigma")
+
+    def set_k2(self, k2):
+        k2 = [_c128(a, s) for a, s in zip(k2, comp_shapes(*self.shape))]
+        _check(lib().aphid_set_k2(self._ctx, *[_p(a) for a in k2]), "aphid_set_k2")
+
+    def set_diag(self, dg):
+        dg = [_c128(a, s) for a, s in zip(dg, comp_shapes(*self.shape))]
+        _check(lib().aphid_set_diag(self._ctx, *[_p(a) for a in dg]), "aphid_set_diag")
+
+    def _zeros4(self):
+        return [np.zeros(s, np.complex128, order="F") for s in comp_shapes(*self.shape)]
+
+    def get_k2(self):
+        out = self._zeros4()
+        _check(lib().aphid_get_k2(self._ctx, *[_p(a) for a in out]), "aphid_get_k2")
+        return out
+
+    def get_diag(self):
+        out = self._zeros4()
+        _check(lib().aphid_get_diag(self._ctx, *[_p(a) for a in out]), "aphid_get_diag")
+        return out
+
+    def setup_case(self, case):
+        """read_mesh + gen_k2_* + gen_diag for a meshgen.Case; returns b."""
+        self.set_mesh(case.xn, case.yn, case.zn)
+        self.set_cells(case.model, case.omega, case.cells)
+        b, skipped = self.build_rhs(case.mode, case.sources)
+        return b
+
+    # -- operator and right-hand side ---------------------------------------
+    def apply(self, u):
+        u = [_c128(a, s) for a, s in zip(u, comp_shapes(*self.shape))]
+        out = self._zeros4()
+        _check(lib().aphid_apply(self._ctx, *[_p(a) for a in u], *[_p(a) for a in out]),
+               "aphid_apply")
+        return out
+
+    def build_rhs(self, mode: int, sources: Sequence[Tuple[int, int, int, int]] = ()):
+        out = self._zeros4()
+        ns = len(sources)
+        arr = np.array(sources, np.int32).reshape(ns, 4) if ns else np.zeros((0, 4), np.int32)
+        cols = [np.ascontiguousarray(arr[:, q]) for q in range(4)]
+        skipped = np.zeros(max(ns, 1), np.int32)
+        _check(lib().aphid_build_rhs(self._ctx, mode, ns, *[_p(a) for a in cols], _p(skipped),
+                                     *[_p(a) for a in out]), "aphid_build_rhs")
+        return out, skipped[:ns]
+
+    # -- solve ------------------------------------------------------------------
+    def qmr_solve(self, b, tol: float, max_it: int, x0=None, flags: int = 0,
+                  hist_cap: Optional[int] = None) -> SolveResult:
+        shp = comp_shapes(*self.shape)
+        x = [np.array(_c128(a, s), order="F", copy=True) for a, s in zip(b, shp)]
+        x0a = [None] * 4
+        if x0 is not None:
+            x0a = [_c128(a, s) for a, s in zip(x0, shp)]
+            flags |= F_X0
+        cap = max_it + 1 if hist_cap is None else hist_cap
+        hist = np.zeros(max(cap, 1))
+        rep = Report()
+        rep.hist = hist.ctypes.data_as(C.POINTER(C.c_double))
+        rep.hist_cap = cap
+        _check(lib().aphid_qmr_solve(self._ctx, *[_p(a) for a in x], *[_p(a) for a in x0a],
+                                     C.c_double(tol), C.c_int(max_it), C.c_uint(flags),
+                                     C.byref(rep)), "aphid_qmr_solve")
+        return SolveResult(x, rep.iters, rep.status, rep.rnorm, rep.rhsnorm, rep.true_rel_res,
+                           rep.seconds_qmr, rep.seconds_total, rep.outer_iters,
+                           hist[:rep.hist_len].copy())
+
+    def load_rhs(self, b, x0=None):
+        shp = comp_shapes(*self.shape)
+        b = [_c128(a, s) for a, s in zip(b, shp)]
+        x0a = [None] * 4 if x0 is None else [_c128(a, s) for a, s in zip(x0, shp)]
+        _check(lib().aphid_load_rhs(self._ctx, *[_p(a) for a in b], *[_p(a) for a in x0a]),
+               "aphid_load_rhs")
+
+    def solve_resident(self, tol: float, max_it: int, flags: int = 0,
+                       hist_cap: int = 0) -> SolveResult:
+        hist = np.zeros(max(hist_cap, 1))
+        rep = Report()
+        rep.hist = hist.ctypes.data_as(C.POINTER(C.c_double))
+        rep.hist_cap = hist_cap
+        _check(lib().aphid_solve_resident(self._ctx, C.c_double(tol), C.c_int(max_it),
+                                          C.c_uint(flags), C.byref(rep)), "aphid_solve_resident")
+        return SolveResult([], rep.iters, rep.status, rep.rnorm, rep.rhsnorm, rep.true_rel_res,
+                           rep.seconds_qmr, rep.seconds_total, rep.outer_iters,
+                           hist[:rep.hist_len].copy())
+
+    def fetch_solution(self):
+        out = self._zeros4()
+        _check(lib().aphid_fetch_solution(self._ctx, *[_p(a) for a in out]),
+               "aphid_fetch_solution")
+        return out
+
+    # -- tuning / timing ------------------------------------------------------
+    def get_tile(self, precision: int = 0):
+        t = [C.c_int(), C.c_int(), C.c_int()]
+        _check(lib().aphid_get_tile(self._ctx, precision, *[C.byref(v) for v in t]), "aphid_get_tile")
+        return tuple(v.value for v in t)
+
+    def set_tile(self, tk: int, tj: int, chunk: int, precision: int = 0):
+        _check(lib().aphid_set_tile(self._ctx, precision, tk, tj, chunk), "aphid_set_tile")
+
+    def time_apply(self, precision: int = 0, reps: int = 20) -> float:
+        s = C.c_double()
+        _check(lib().aphid_time_apply(self._ctx, precision, reps, C.byref(s)), "aphid_time_apply")
+        return s.value
+
+    def time_iteration(self, precision: int = 0, reps: int = 10):
+        """[p-update, stencil+eps, ||t-beta v||, fused update, iteration] seconds."""
+        out = np.zeros(5)
+        _check(lib().aphid_time_iteration(self._ctx, precision, reps, _p(out)),
+               "aphid_time_iteration")
+        return out
+
+
+EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_set_mesh",
+           "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
+           "aphid_get_diag", "aphid_apply", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
+           "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
+           "aphid_fetch_solution", "aphid_time_iteration"]

@@ -1,0 +1,163 @@
+/*
+ * aphid_gpu.h -- C ABI of the MI355X A-Phi solve (libaphid_gpu.so).
+ *
+ * Drop-in for the reference's hot path: the matrix-free operator
+ * (inprod_x/y/z/p, src/aphid.f90:803-1282), its setup (gen_k2_*grid
+ * :1289-1535, gen_diag :1542-1725, gen_inprod_*_coefs :750-1185, the
+ * sigma-hat models of read_mesh :379-449, gen_rhs :534-743) and the
+ * Jacobi-preconditioned complex-symmetric QMR (qmr / qmr_mthread,
+ * src/aphid_qmr.f90:5-972).  In the reference these communicate through
+ * module globals (src/aphid_mod.f90:5-75); here they take explicit
+ * arguments.  A Fortran caller binds them with iso_c_binding
+ * (aphid_git_amd/fortran/aphid_gpu_mod.f90).
+ *
+ * Conventions
+ *  - nx, ny, nz are NODE counts (src/aphid_mod.f90:39).
+ *  - Host arrays are caller-owned, in Fortran order (i fastest) with the
+ *    reference's shapes: x-edges (nx-1,ny,nz), y-edges (nx,ny-1,nz),
+ *    z-edges (nx,ny,nz-1), nodes (nx,ny,nz), cells (nx-1,ny-1,nz-1).
+ *  - Complex arrays are interleaved (re,im) doubles, i.e. Fortran
+ *    complex(c_double_complex) / C double _Complex.
+ *  - Return values: 0 success (for a solve: converged), 1 max_it reached
+ *    (the iterate is still returned, as the reference does), 2 breakdown or
+ *    a non-finite value; negative = error (APHID_E_*), with a message from
+ *    aphid_last_error().  No entry point exits or aborts.
+ *  - One context per host thread; calls are synchronous; the library owns
+ *    all device memory of a context.  The device layout is private (z
+ *    fastest); the Fortran<->device permutation happens inside the calls.
+ */
+#ifndef APHID_GPU_H
+#define APHID_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APHID_OK 0
+#define APHID_MAXIT 1
+#define APHID_BREAKDOWN 2
+#define APHID_E_ARG (-1)
+#define APHID_E_HIP (-2)
+#define APHID_E_STATE (-3)
+#define APHID_E_NOMEM (-4)
+
+/* aphid_qmr_solve flags */
+#define APHID_F_X0 1u          /* x0 arrays hold an initial guess (else x0 = 0) */
+#define APHID_F_NO_TRUE_RES 2u /* skip the final ||b - A x|| check */
+#define APHID_F_MIXED 4u       /* complex64 Krylov vectors + FP64 scalars and
+                                  FP64 iterative refinement (see report) */
+
+typedef struct aphid_ctx aphid_ctx;
+
+typedef struct {
+  int iters;            /* the reference's `iter` (max_it+1 when exhausted) */
+  int status;           /* APHID_OK / APHID_MAXIT / APHID_BREAKDOWN */
+  double rnorm;         /* final recurrence residual norm ||r|| */
+  double rhsnorm;       /* ||b|| */
+  double true_rel_res;  /* ||b - A x|| / ||b|| recomputed in complex128 */
+  double seconds_qmr;   /* device time of the iteration loop */
+  double seconds_total; /* wall time of the call (H2D, setup, solve, D2H) */
+  double *hist;         /* optional, caller-owned: ||r|| per iteration */
+  int hist_cap;         /* capacity of hist (0 = none) */
+  int hist_len;         /* out: entries written */
+  int outer_iters;      /* out: refinement sweeps (APHID_F_MIXED), else 1 */
+  int reserved;
+} aphid_qmr_report;
+
+/* Context life cycle.  device < 0 selects the current HIP device. */
+int aphid_ctx_create(aphid_ctx **ctx, int device);
+int aphid_ctx_destroy(aphid_ctx *ctx);
+const char *aphid_last_error(void);
+
+/* Mesh from node coordinates: metrics dx(0:n), dxc(1:n) (read_mesh,
+ * src/aphid.f90:341-361) and the operator's 1-D coefficient tables
+ * (gen_inprod_{x,y,z,p}_coefs, src/aphid.f90:750-797,878-925,1007-1054,
+ * 1135-1185).  Requires nx, ny, nz >= 3. */
+int aphid_set_mesh(aphid_ctx *ctx, int nx, int ny, int nz, const double *xn,
+                   const double *yn, const double *zn);
+
+/* Cell parameters -> sigma-hat on the device (read_mesh's models,
+ * src/aphid.f90:379-449: 'R' a=sigma; 'C' a=sigma,b=eps_rel;
+ * 'D' a=sigma,b=eta,c=tau,d=cee; 'I' b=eps_rel; unused pointers may be
+ * NULL), then k^2 grids (gen_k2_*grid) and the Jacobi diagonal (gen_diag)
+ * for angular frequency omega. */
+int aphid_set_cells(aphid_ctx *ctx, char model, double omega, const float *a,
+                    const float *b, const float *c, const float *d);
+
+/* Complex cell sigma-hat (Fortran-order (nx-1,ny-1,nz-1)) -> k^2 grids and
+ * the Jacobi diagonal on the device (the reference's `sig` module array). */
+int aphid_set_sigma(aphid_ctx *ctx, const double *sig_c, double omega);
+
+/* Upload k^2 grids directly (module k2_*grid); rebuilds the diagonal. */
+int aphid_set_k2(aphid_ctx *ctx, const double *k2x_c, const double *k2y_c,
+                 const double *k2z_c, const double *k2p_c);
+
+/* Override the Jacobi diagonal (module diag_*). */
+int aphid_set_diag(aphid_ctx *ctx, const double *dgx_c, const double *dgy_c,
+                   const double *dgz_c, const double *dgp_c);
+
+/* Download the device k^2 grids / diagonal (for checks). */
+int aphid_get_k2(aphid_ctx *ctx, double *k2x_c, double *k2y_c, double *k2z_c,
+                 double *k2p_c);
+int aphid_get_diag(aphid_ctx *ctx, double *dgx_c, double *dgy_c, double *dgz_c,
+                   double *dgp_c);
+
+/* out = K in over all four row blocks: inprod_x, inprod_y, inprod_z,
+ * inprod_p (src/aphid.f90:803-1282).  Inputs and outputs may not alias. */
+int aphid_apply(aphid_ctx *ctx, const double *in_x, const double *in_y,
+                const double *in_z, const double *in_p, double *out_x,
+                double *out_y, double *out_z, double *out_p);
+
+/* Right-hand side (gen_rhs, src/aphid.f90:534-743) into caller arrays.
+ * Mode 5 takes nsrc dipoles with dir in {+1,-1,+2,-2,+3,-3} = +X..-Z and a
+ * 1-based tail node; skipped[s] (optional) is set to 1 for a dipole the
+ * reference would skip.  Host-side; needs only aphid_set_mesh. */
+int aphid_build_rhs(aphid_ctx *ctx, int mode, int nsrc, const int *dir,
+                    const int *ip, const int *jp, const int *kp, int *skipped,
+                    double *bx_c, double *by_c, double *bz_c, double *bp_c);
+
+/* Jacobi-QMR solve of K x = b (qmr / qmr_mthread, src/aphid_qmr.f90).
+ * x_* hold b on entry and the solution on exit (the reference's in-place
+ * rhs_* convention).  With APHID_F_X0, x0_* hold the initial guess
+ * (src/aphid_qmr.f90:245-258 already forms r0 = b - A x0); the stop test
+ * stays ||r|| < tol*||b|| (:386). */
+int aphid_qmr_solve(aphid_ctx *ctx, double *x_x, double *x_y, double *x_z,
+                    double *x_p, const double *x0_x, const double *x0_y,
+                    const double *x0_z, const double *x0_p, double tol,
+                    int max_it, unsigned flags, aphid_qmr_report *rep);
+
+/* Device-resident variant of aphid_qmr_solve for callers that solve many
+ * right-hand sides or time the solve with its inputs already in HBM:
+ * aphid_load_rhs uploads b (and x0 when given; else x0 = 0),
+ * aphid_solve_resident runs the same solve on the resident vectors and
+ * leaves x on the device, aphid_fetch_solution downloads it.  Together they
+ * are aphid_qmr_solve split at its host transfers. */
+int aphid_load_rhs(aphid_ctx *ctx, const double *bx_c, const double *by_c,
+                   const double *bz_c, const double *bp_c, const double *x0_x,
+                   const double *x0_y, const double *x0_z, const double *x0_p);
+int aphid_solve_resident(aphid_ctx *ctx, double tol, int max_it, unsigned flags,
+                         aphid_qmr_report *rep);
+int aphid_fetch_solution(aphid_ctx *ctx, double *x_x, double *x_y, double *x_z,
+                         double *x_p);
+
+/* Stencil launch shape in use (set by the autotuner on aphid_set_mesh, or
+ * fixed by aphid_set_tile): tile along z (tk), along y (tj), x planes per
+ * block (chunk). precision: 0 = complex128 kernel, 1 = complex64 kernel. */
+int aphid_get_tile(aphid_ctx *ctx, int precision, int *tk, int *tj, int *chunk);
+int aphid_set_tile(aphid_ctx *ctx, int precision, int tk, int tj, int chunk);
+
+/* Timing hooks for benchmarks: run `reps` device-resident operator applies
+ * (complex128 if precision == 0, complex64 if 1) on the context's current
+ * vectors and return the mean kernel time in seconds. */
+int aphid_time_apply(aphid_ctx *ctx, int precision, int reps, double *seconds);
+
+/* Mean device time (seconds) of each kernel of one QMR iteration, timed on
+ * the context's vectors after a solve: secs[0] p-update, [1] stencil with
+ * the eps reduction, [2] ||t - beta v||, [3] fused update, [4] the whole
+ * iteration.  Overwrites the resident iterate. */
+int aphid_time_iteration(aphid_ctx *ctx, int precision, int reps, double *secs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

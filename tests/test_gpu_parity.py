@@ -1,0 +1,206 @@
+"""HIP path (through the C ABI) against the CPU oracle and the reference's
+golden vectors.  Floating-point tolerances are written in each test: the
+setup kernels follow the reference's operation order (1e-14), the stencil
+uses fused multiply-adds (1e-13 relative to the component's max), and the
+solve must reproduce the reference's complex128 fields within 1e-7."""
+import os
+
+import numpy as np
+import pytest
+
+from aphid_git_amd import meshgen
+from conftest import flat4, golden_files, load_golden, relerr, shapes, split4
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from aphid_git_amd import gpu as G
+
+    s = G.Solver()
+    yield s
+    s.close()
+
+
+def _setup(gpu, case):
+    gpu.set_mesh(case.xn, case.yn, case.zn)
+    gpu.set_cells(case.model, case.omega, case.cells)
+
+
+def _rand4(shape, seed):
+    rng = np.random.default_rng(seed)
+    return [np.asfortranarray(rng.standard_normal(s) + 1j * rng.standard_normal(s))
+            for s in shapes(*shape)]
+
+
+CASES = {
+    "tiny_R": lambda: meshgen.tiny_case(7, 6, 5, seed=1, model="R", freq=3.0),
+    "tiny_C": lambda: meshgen.tiny_case(9, 8, 7, seed=2, model="C", freq=2.0e3),
+    "tiny_D": lambda: meshgen.tiny_case(8, 7, 9, seed=3, model="D", freq=50.0),
+    "tiny_I": lambda: meshgen.tiny_case(6, 7, 8, seed=4, model="I", freq=1.0e6),
+    "blocks_23x31x19": lambda: meshgen.random_block_case((22, 30, 18), seed=5, freq=30.0),
+    "config1": meshgen.config1,
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_k2_and_diag_match_oracle(gpu, oracle, name):
+    case = CASES[name]()
+    print(f"[k2/diag] {name} {case.shape}", flush=True)
+    _setup(gpu, case)
+    S = oracle.System(case)
+    for got, want in zip(gpu.get_k2(), S.k2):
+        assert relerr(got, want) < 1e-14
+    for got, want in zip(gpu.get_diag(), S.diag):
+        assert relerr(got, want) < 1e-14
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_apply_matches_oracle(gpu, oracle, name):
+    case = CASES[name]()
+    print(f"[apply] {name} {case.shape}", flush=True)
+    _setup(gpu, case)
+    S = oracle.System(case)
+    for seed in range(2):
+        u = _rand4(case.shape, seed)
+        got, want = gpu.apply(u), S.op.apply(u)
+        for c in range(4):
+            assert relerr(got[c], want[c]) < 1e-13, f"component {c}"
+
+
+@pytest.mark.parametrize("path", golden_files("tiny_"), ids=os.path.basename)
+def test_apply_matches_reference_goldens(gpu, path):
+    case, z = load_golden(path)
+    print(f"[golden apply] {os.path.basename(path)}", flush=True)
+    _setup(gpu, case)
+    nx, ny, nz = case.shape
+    u, n = split4(z["apply_in"], nx, ny, nz)
+    ref, _ = split4(z["apply_out"], nx, ny, nz)
+    got = gpu.apply(u)
+    for c in range(4):
+        assert relerr(got[c], ref[c]) < 1e-13
+    for c, name in enumerate("xyzp"):
+        assert relerr(gpu.get_k2()[c].ravel(order="F"), z[f"k2_{name}"]) < 1e-14
+        assert relerr(gpu.get_diag()[c].ravel(order="F"), z[f"diag_{name}"]) < 1e-14
+
+
+def test_every_tile_shape_gives_identical_results(gpu):
+    case = CASES["blocks_23x31x19"]()
+    _setup(gpu, case)
+    u = _rand4(case.shape, 7)
+    ref = None
+    for tk, tj in [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4)]:
+        for chunk in (1, 3, 8, 64):
+            print(f"[tile] {tk}x{tj} chunk {chunk}", flush=True)
+            gpu.set_tile(tk, tj, chunk)
+            out = flat4(gpu.apply(u))
+            if ref is None:
+                ref = out
+            else:
+                np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("path", golden_files("rhs_"), ids=os.path.basename)
+def test_rhs_matches_reference(gpu, path):
+    case, z = load_golden(path)
+    gpu.set_mesh(case.xn, case.yn, case.zn)
+    b, skipped = gpu.build_rhs(case.mode, case.sources)
+    for c, name in enumerate("xyzp"):
+        flat = b[c].ravel(order="F")
+        if f"rhs_{name}" in z:
+            assert np.array_equal(flat, z[f"rhs_{name}"])
+        else:
+            idx = z[f"rhsnz_{name}_idx"]
+            assert np.array_equal(np.nonzero(flat)[0], idx)
+            assert np.array_equal(flat[idx], z[f"rhsnz_{name}_val"])
+    assert skipped.tolist() == z["skipped"].tolist()
+
+
+@pytest.mark.parametrize("path", [p for p in golden_files("tiny_") if "qmr_hist" in np.load(p)],
+                         ids=os.path.basename)
+def test_qmr_matches_reference_goldens(gpu, path):
+    case, z = load_golden(path)
+    print(f"[qmr golden] {os.path.basename(path)}", flush=True)
+    b = gpu.setup_case(case)
+    r = gpu.qmr_solve(b, case.target, case.max_it)
+    assert r.status == 0
+    ref_hist = z["qmr_hist"]
+    assert np.max(np.abs(r.hist[:20] - ref_hist[:20]) / ref_hist[:20]) < 1e-10
+    assert abs(r.iters - int(z["qmr_iters"])) <= max(3, 0.1 * int(z["qmr_iters"]))
+    sol = flat4(r.x)
+    assert np.linalg.norm(sol - z["qmr_sol"]) / np.linalg.norm(z["qmr_sol"]) < 1e-7
+
+
+def test_config1_solve_matches_reference_and_oracle(gpu, oracle):
+    """32^3 air/half-space, 10 Hz, rel-res 1e-8: the reference takes 364
+    iterations.  Fields within 1e-7 (relative, 2-norm over all unknowns) of
+    the oracle's complex128 solve and of the reference's sampled entries."""
+    case, z = load_golden(golden_files("config1_")[0])
+    b = gpu.setup_case(case)
+    r = gpu.qmr_solve(b, case.target, case.max_it)
+    print(f"[config1] iters {r.iters} rnorm/b {r.rnorm / r.rhsnorm:.3e} true {r.true_rel_res:.3e}",
+          flush=True)
+    assert r.status == 0
+    assert abs(r.iters - int(z["qmr_iters"])) <= 0.05 * int(z["qmr_iters"])
+    assert np.max(np.abs(r.hist[:30] - z["qmr_hist"][:30]) / z["qmr_hist"][:30]) < 1e-10
+    assert r.true_rel_res < 1e-8 * 1.01
+    sol = flat4(r.x)
+    idx, val = z["qmr_sol_idx"], z["qmr_sol_val"]
+    assert np.linalg.norm(sol[idx] - val) / np.linalg.norm(val) < 1e-7
+    S = oracle.System(case)
+    xo, rep, hist = S.solve()
+    xo = flat4(xo)
+    assert np.linalg.norm(sol - xo) / np.linalg.norm(xo) < 1e-7
+
+
+def test_warm_start_from_a_solution(gpu):
+    case = CASES["blocks_23x31x19"]()
+    b = gpu.setup_case(case)
+    r = gpu.qmr_solve(b, 1e-10, 5000)
+    assert r.status == 0
+    r2 = gpu.qmr_solve(b, 1e-8, 5000, x0=r.x)
+    # r0 = b - K x0 is already below 1e-8 ||b||: QMR stops after one step
+    assert r2.status == 0 and r2.iters <= 2
+    assert r2.true_rel_res < 1e-8
+
+
+def test_max_it_semantics_and_history(gpu):
+    case = CASES["blocks_23x31x19"]()
+    b = gpu.setup_case(case)
+    r = gpu.qmr_solve(b, 1e-30, 17)
+    assert r.status == 1 and r.iters == 18 and len(r.hist) == 17
+    assert np.all(np.diff(r.hist) != 0)
+
+
+def test_zero_rhs_and_breakdown_are_statuses(gpu):
+    case = meshgen.tiny_case(5, 5, 5, seed=2, model="I", freq=0.0)
+    b = gpu.setup_case(case)
+    r = gpu.qmr_solve(b, 1e-8, 50)
+    assert r.status == 2
+    case = CASES["tiny_R"]()
+    b = gpu.setup_case(case)
+    r = gpu.qmr_solve([np.zeros_like(a) for a in b], 1e-8, 50)
+    assert r.status == 0 and r.iters == 0
+
+
+def test_full_size_config2_operator_symmetry_and_solve(gpu):
+    """128x128x96 cells (the benchmark mesh): size-independent checks -- K is
+    complex symmetric on the interior-Phi subspace (u^T K v = v^T K u), and
+    the solve's true residual meets rel-res 1e-8."""
+    case = meshgen.config2()
+    print(f"[config2] {case.shape} M={case.n_unknowns}", flush=True)
+    b = gpu.setup_case(case)
+    u, v = _rand4(case.shape, 1), _rand4(case.shape, 2)
+    for w in (u, v):
+        w[3][0, :, :] = w[3][-1, :, :] = 0
+        w[3][:, 0, :] = w[3][:, -1, :] = 0
+        w[3][:, :, 0] = w[3][:, :, -1] = 0
+    Ku, Kv = flat4(gpu.apply(u)), flat4(gpu.apply(v))
+    a, bb = flat4(v) @ Ku, flat4(u) @ Kv
+    assert abs(a - bb) <= 1e-11 * np.linalg.norm(flat4(u)) * np.linalg.norm(Kv)
+    r = gpu.qmr_solve(b, 1e-8, 20000)
+    print(f"[config2] iters {r.iters} status {r.status} true {r.true_rel_res:.3e} "
+          f"t {r.seconds_qmr:.3f}s", flush=True)
+    assert r.status == 0
+    assert r.true_rel_res < 1.05e-8
