@@ -34,7 +34,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 256;
-constexpr int kRedBlocks = 1024;        // grid of the reduction kernels
+constexpr int kRedBlocks = 2048;        // grid of the reduction kernels
 constexpr int kMaxPartBlocks = 1 << 16; // capacity of the partial-sum buffer
 
 // ---------------------------------------------------------------------------
@@ -143,7 +143,6 @@ struct aphid_ctx {
   double* hist = nullptr;
   int hist_cap = 0;
   double* resbuf = nullptr;  // [||b-Ax||, ||b||]
-  unsigned* ticket2 = nullptr;
   // graph of `giters` iterations per precision
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int giters[2] = {0, 0};
@@ -373,14 +372,20 @@ int enqueue_iteration(aphid_ctx* c) {
   const Vecs<V> w = vecs<V>(c);
   const long long n4 = 4 * c->nn;
   const cplx<V>* invd = invd_of<V>(c);
-  hipLaunchKernelGGL(k_pupdate<V>, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, w.v, invd, w.p, n4,
+  hipLaunchKernelGGL((k_pupdate<V, 4>), dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.v, invd, w.p, n4,
                      c->st);
   ApplyEpilogue ep{c->st, c->part};
-  CK(launch_apply<V>(c->tile[prec], opargs<V>(c), w.p, w.t, ep, c->s));
+  const Tile& tl = c->tile[prec];
+  CK(launch_apply<V>(tl, opargs<V>(c), w.p, w.t, ep, c->s));
+  hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
+                     (int)apply_blocks(tl, c->nx, c->ny, c->nz), c->st);
   hipLaunchKernelGGL(k_vtnorm<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.t, w.v, n4, c->st,
                      c->part);
+  hipLaunchKernelGGL(k_fin_vtnorm, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st);
   hipLaunchKernelGGL(k_update<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.p, w.d, w.x, w.t, w.s,
-                     w.r, w.v, invd, n4, c->st, c->part, c->hist);
+                     w.r, w.v, invd, n4, c->st, c->part);
+  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st,
+                     c->hist);
   CK(hipGetLastError());
   return 0;
 }
@@ -437,19 +442,20 @@ int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs)
   init.max_it = max_it;
   init.hist_cap = c->hist_cap;
   init.done = 0;
-  init.ticket = 0;
   CK(hipMemcpyAsync(c->st, &init, sizeof init, hipMemcpyHostToDevice, c->s));
   CK(hipEventRecord(c->ev0, c->s));
-  hipLaunchKernelGGL(k_norm_b<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.b, n4, c->st, c->part,
-                     tol);
+  hipLaunchKernelGGL(k_norm_b<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.b, n4, c->part);
+  hipLaunchKernelGGL(k_fin_norm_b, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st, tol);
   if (have_x0) {
     ApplyEpilogue ep{nullptr, nullptr};
     CK(launch_apply<V>(c->tile[prec], opargs<V>(c), w.x, w.t, ep, c->s));
   }
   hipLaunchKernelGGL(k_init_r<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.b,
-                     have_x0 ? (const cplx<V>*)w.t : nullptr, w.r, n4, c->st, c->part);
+                     have_x0 ? (const cplx<V>*)w.t : nullptr, w.r, n4, c->part);
+  hipLaunchKernelGGL(k_fin_init_r, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st);
   hipLaunchKernelGGL(k_init_v<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.r, invd_of<V>(c), w.v,
                      w.p, w.d, w.s, n4, c->st, c->part);
+  hipLaunchKernelGGL(k_fin_init_v, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st);
   CK(hipGetLastError());
   CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
   CK(hipStreamSynchronize(c->s));
@@ -469,7 +475,9 @@ int true_residual(aphid_ctx* c, double* rel, double* rnorm, double* bnorm) {
   ApplyEpilogue ep{nullptr, nullptr};
   CK(launch_apply<double>(c->tile[0], c->op64, c->V[0], c->V[8], ep, c->s));
   hipLaunchKernelGGL(k_resnorm<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, c->V[1], c->V[8],
-                     4 * c->nn, c->resbuf, c->part, c->ticket2);
+                     4 * c->nn, c->part);
+  hipLaunchKernelGGL(k_fin_resnorm, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks,
+                     c->resbuf);
   CK(hipGetLastError());
   double h[2];
   CK(hipMemcpyAsync(h, c->resbuf, sizeof h, hipMemcpyDeviceToHost, c->s));
@@ -477,6 +485,74 @@ int true_residual(aphid_ctx* c, double* rel, double* rnorm, double* bnorm) {
   *rnorm = h[0];
   *bnorm = h[1];
   *rel = h[1] > 0 ? h[0] / h[1] : h[0];
+  return 0;
+}
+
+// Mixed precision (APHID_F_MIXED): FP64 iterative refinement on the true
+// residual around complex64 QMR solves.  x (V[0]) and b (V[1]) stay
+// complex128; each sweep solves K e = (b - K x)/||b - K x|| with the
+// complex64 stencil and vectors (FP64 inner products and scalars) to a
+// relative residual inner_tol, then x += ||b - K x|| e in FP64.
+double mixed_inner_floor() {
+  const char* e = getenv("APHID_MIXED_INNER_TOL");
+  const double v = e ? atof(e) : 0.0;
+  return v > 0.0 ? v : 1e-5;
+}
+
+int solve_mixed(aphid_ctx* c, double tol, int max_it, bool have_x0, QmrState* out,
+                int* sweeps_out, std::vector<double>* sweep_res, double* secs) {
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = ensure_vectors(c, true);
+  if (rc) return rc;
+  rc = autotune(c, 1);
+  if (rc) return rc;
+  const long long n4 = 4 * c->nn;
+  if (!have_x0) CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
+  const double floor_tol = mixed_inner_floor();
+  int total = 0, sweeps = 0, status = ST_MAXIT;
+  double rel = NAN, rn = NAN, bn = NAN;
+  QmrState last{};
+  for (;;) {
+    rc = true_residual(c, &rel, &rn, &bn);  // t = K x in V[8]; resbuf = [||b-Kx||, ||b||]
+    if (rc) return rc;
+    sweep_res->push_back(rel);
+    if (rn < tol * bn || bn == 0.0) {
+      status = ST_CONV;
+      break;
+    }
+    if (!std::isfinite(rn)) {
+      status = ST_BREAK;
+      break;
+    }
+    if (total >= std::max(max_it, 1) || sweeps >= 64) break;
+    hipLaunchKernelGGL(k_inner_rhs, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, c->V[1], c->V[8],
+                       c->resbuf, c->F[1], n4);
+    CK(hipGetLastError());
+    CK(hipMemsetAsync(c->F[0], 0, sizeof(c64) * n4, c->s));
+    const double inner_tol = std::max(0.5 * tol * bn / rn, floor_tol);
+    double s_in = 0.0;
+    rc = qmr_device<float>(c, inner_tol, std::max(max_it - total, 1), false, &s_in);
+    if (rc) return rc;
+    last = *c->hst;
+    const int executed = last.status == ST_MAXIT ? std::max(max_it - total, 1) : last.iter;
+    total += std::max(executed, 0);
+    if (last.status == ST_BREAK && executed <= 1) {
+      status = ST_BREAK;
+      break;
+    }
+    hipLaunchKernelGGL(k_refine, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, c->V[0], c->F[0],
+                       c->resbuf, n4);
+    CK(hipGetLastError());
+    ++sweeps;
+  }
+  CK(hipStreamSynchronize(c->s));
+  *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out = last;
+  out->iter = status == ST_MAXIT ? std::max(max_it, 1) + 1 : total;
+  out->status = status;
+  out->rnorm = rn;
+  out->rhsnorm = bn;
+  *sweeps_out = sweeps;
   return 0;
 }
 
@@ -506,9 +582,7 @@ int aphid_ctx_create(aphid_ctx** out, int device) {
   CK(hipHostMalloc(&c->hst, sizeof(QmrState), hipHostMallocDefault));
   CK(hipMalloc(&c->part, sizeof(double) * 4 * kMaxPartBlocks));
   CK(hipMalloc(&c->resbuf, sizeof(double) * 4));
-  CK(hipMalloc(&c->ticket2, sizeof(unsigned) * 2));
   CK(hipMemset(c->st, 0, sizeof(QmrState)));
-  CK(hipMemset(c->ticket2, 0, sizeof(unsigned) * 2));
   CK(hipEventCreate(&c->ev0));
   CK(hipEventCreate(&c->ev1));
   *out = c;
@@ -524,7 +598,6 @@ int aphid_ctx_destroy(aphid_ctx* c) {
   if (c->hst) hipHostFree(c->hst);
   if (c->part) hipFree(c->part);
   if (c->resbuf) hipFree(c->resbuf);
-  if (c->ticket2) hipFree(c->ticket2);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->s) hipStreamDestroy(c->s);
@@ -726,6 +799,34 @@ int aphid_apply(aphid_ctx* c, const double* ix, const double* iy, const double* 
   return download4(c, c->V[4], hout);
 }
 
+int aphid_apply_prec(aphid_ctx* c, int precision, const double* ix, const double* iy,
+                     const double* iz, const double* ip, double* ox, double* oy, double* oz,
+                     double* op) {
+  if (precision == 0) return aphid_apply(c, ix, iy, iz, ip, ox, oy, oz, op);
+  if (precision != 1) return fail(APHID_E_ARG, "precision must be 0 or 1");
+  if (!c || !ix || !iy || !iz || !ip || !ox || !oy || !oz || !op)
+    return fail(APHID_E_ARG, "NULL argument");
+  if (!c->have_k2) return fail(APHID_E_STATE, "set k2 (sigma, cells or k2) first");
+  CK(hipSetDevice(c->dev));
+  int rc = ensure_vectors(c, true);
+  if (rc) return rc;
+  rc = autotune(c, 1);
+  if (rc) return rc;
+  const double* hin[4] = {ix, iy, iz, ip};
+  rc = upload4(c, hin, c->V[3]);
+  if (rc) return rc;
+  const long long n4 = 4 * c->nn;
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(n4)), dim3(kBlock), 0, c->s, c->V[3],
+                     c->F[3], n4);
+  ApplyEpilogue ep{nullptr, nullptr};
+  CK(launch_apply<float>(c->tile[1], c->op32, c->F[3], c->F[4], ep, c->s));
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(n4)), dim3(kBlock), 0, c->s, c->F[4],
+                     c->V[4], n4);
+  CK(hipGetLastError());
+  double* hout[4] = {ox, oy, oz, op};
+  return download4(c, c->V[4], hout);
+}
+
 int aphid_build_rhs(aphid_ctx* c, int mode, int nsrc, const int* dir, const int* ip_,
                     const int* jp_, const int* kp_, int* skipped, double* bx, double* by,
                     double* bz, double* bp) {
@@ -871,11 +972,33 @@ int aphid_solve_resident(aphid_ctx* c, double tol, int max_it, unsigned flags,
   if (!c->have_k2 || !c->have_diag) return fail(APHID_E_STATE, "operator not set up");
   if (!c->V[1]) return fail(APHID_E_STATE, "aphid_load_rhs first");
   if (!(tol >= 0.0) || max_it < 0) return fail(APHID_E_ARG, "tol and max_it must be >= 0");
-  if (flags & APHID_F_MIXED) return fail(APHID_E_ARG, "mixed precision not available in this build");
   const bool have_x0 = flags & APHID_F_X0;
   CK(hipSetDevice(c->dev));
   int rc = autotune(c, 0);
   if (rc) return rc;
+  if (flags & APHID_F_MIXED) {
+    QmrState hs{};
+    int sweeps = 0;
+    std::vector<double> res;
+    double secs = 0.0;
+    rc = solve_mixed(c, tol, max_it, have_x0, &hs, &sweeps, &res, &secs);
+    if (rc) return rc;
+    if (rep) {
+      rep->iters = hs.iter;
+      rep->status = hs.status;
+      rep->rnorm = hs.rnorm;
+      rep->rhsnorm = hs.rhsnorm;
+      rep->true_rel_res = res.empty() ? NAN : res.back();
+      rep->seconds_qmr = secs;
+      rep->outer_iters = sweeps;
+      const int cap = (rep->hist && rep->hist_cap > 0) ? rep->hist_cap : 0;
+      rep->hist_len = std::min<int>((int)res.size(), cap);
+      for (int q = 0; q < rep->hist_len; ++q) rep->hist[q] = res[q];
+      rep->seconds_total =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return hs.status;
+  }
   const int cap = (rep && rep->hist && rep->hist_cap > 0) ? rep->hist_cap : 0;
   if (cap > c->hist_cap) {
     if (c->hist) CK(hipFree(c->hist));
@@ -995,27 +1118,31 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   hs.iter = 1;
   hs.max_it = 1 << 30;
   hs.target = 0.0;
-  hs.ticket = 0;
   if (!(hs.rho_np > 0.0) || !std::isfinite(hs.rho_np) || is_zero(hs.del))
     return fail(APHID_E_STATE, "the last solve left no usable Krylov state");
   CK(hipMemcpyAsync(c->st, &hs, sizeof hs, hipMemcpyHostToDevice, c->s));
   const Vecs<double> w = vecs<double>(c);
   const long long n4 = 4 * c->nn;
   auto K0 = [&]() {
-    hipLaunchKernelGGL(k_pupdate<double>, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, w.v, c->invd,
-                       w.p, n4, c->st);
+    hipLaunchKernelGGL((k_pupdate<double, 4>), dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.v,
+                       c->invd, w.p, n4, c->st);
   };
   auto K1 = [&]() {
     ApplyEpilogue ep{c->st, c->part};
     launch_apply<double>(c->tile[0], c->op64, w.p, w.t, ep, c->s);
+    hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
+                       (int)apply_blocks(c->tile[0], c->nx, c->ny, c->nz), c->st);
   };
   auto K2 = [&]() {
     hipLaunchKernelGGL(k_vtnorm<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.t, w.v, n4,
                        c->st, c->part);
+    hipLaunchKernelGGL(k_fin_vtnorm, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st);
   };
   auto K3 = [&]() {
     hipLaunchKernelGGL(k_update<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.p, w.d, w.x,
-                       w.t, w.s, w.r, w.v, c->invd, n4, c->st, c->part, nullptr);
+                       w.t, w.s, w.r, w.v, c->invd, n4, c->st, c->part);
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st,
+                       nullptr);
   };
   float ms[5] = {0, 0, 0, 0, 0};
   for (int rep = 0; rep < reps + 1; ++rep) {  // first sweep warms up

@@ -28,12 +28,16 @@ __device__ __forceinline__ vec<V> dn(c128 a) { return {(V)a.re, (V)a.im}; }
 
 // ||b||^2 and the stop target (src/aphid_qmr.f90:169-172,386)
 template <typename V>
-__global__ void k_norm_b(const vec<V>* __restrict__ b, long long n, QmrState* st, double* part,
-                         double tol) {
-  double v[1] = {0.0}, tot[1];
+__global__ void k_norm_b(const vec<V>* __restrict__ b, long long n, double* part) {
+  double v[1] = {0.0};
   GRID_LOOP(q, n) v[0] += abs2(up(b[q]));
   block_sum<1>(v);
-  if (grid_finish<1>(v, part, &st->ticket, tot) && threadIdx.x == 0) {
+  block_partial<1>(v, part);
+}
+__global__ void k_fin_norm_b(const double* part, int nb, QmrState* st, double tol) {
+  double tot[1];
+  finalize_sum<1>(part, nb, tot);
+  if (threadIdx.x == 0) {
     st->rhsnorm = sqrt(tot[0]);
     st->target = tol * st->rhsnorm;
   }
@@ -42,8 +46,8 @@ __global__ void k_norm_b(const vec<V>* __restrict__ b, long long n, QmrState* st
 // r = b - t (t = K x0, or nothing when x0 = 0), rho = ||r|| (:245-273)
 template <typename V>
 __global__ void k_init_r(const vec<V>* __restrict__ b, const vec<V>* __restrict__ t,
-                         vec<V>* __restrict__ r, long long n, QmrState* st, double* part) {
-  double v[1] = {0.0}, tot[1];
+                         vec<V>* __restrict__ r, long long n, double* part) {
+  double v[1] = {0.0};
   GRID_LOOP(q, n) {
     c128 e = up(b[q]);
     if (t) e = e - up(t[q]);
@@ -52,7 +56,12 @@ __global__ void k_init_r(const vec<V>* __restrict__ b, const vec<V>* __restrict_
     v[0] += abs2(up(ev));
   }
   block_sum<1>(v);
-  if (grid_finish<1>(v, part, &st->ticket, tot) && threadIdx.x == 0) st->rho_np = sqrt(tot[0]);
+  block_partial<1>(v, part);
+}
+__global__ void k_fin_init_r(const double* part, int nb, QmrState* st) {
+  double tot[1];
+  finalize_sum<1>(part, nb, tot);
+  if (threadIdx.x == 0) st->rho_np = sqrt(tot[0]);
 }
 
 // v = r/rho; p = d = s = 0; del = v^T M^-1 v; the loop-entry scalars
@@ -60,9 +69,9 @@ __global__ void k_init_r(const vec<V>* __restrict__ b, const vec<V>* __restrict_
 template <typename V>
 __global__ void k_init_v(const vec<V>* __restrict__ r, const vec<V>* __restrict__ invd,
                          vec<V>* __restrict__ v, vec<V>* __restrict__ p, vec<V>* __restrict__ d,
-                         vec<V>* __restrict__ s, long long n, QmrState* st, double* part) {
+                         vec<V>* __restrict__ s, long long n, const QmrState* st, double* part) {
   const double rho = st->rho_np;
-  double a[2] = {0.0, 0.0}, tot[2];
+  double a[2] = {0.0, 0.0};
   const vec<V> z = {V(0), V(0)};
   GRID_LOOP(q, n) {
     const vec<V> vv = dn<V>(divr(up(r[q]), rho));
@@ -77,7 +86,13 @@ __global__ void k_init_v(const vec<V>* __restrict__ r, const vec<V>* __restrict_
     a[1] += pr.im;
   }
   block_sum<2>(a);
-  if (grid_finish<2>(a, part, &st->ticket, tot) && threadIdx.x == 0) {
+  block_partial<2>(a, part);
+}
+__global__ void k_fin_init_v(const double* part, int nb, QmrState* st) {
+  double tot[2];
+  finalize_sum<2>(part, nb, tot);
+  if (threadIdx.x == 0) {
+    const double rho = st->rho_np;
     st->c_n = 1.0;
     st->eps = {1.0, 0.0};
     st->theta_n = 0.0;
@@ -103,25 +118,62 @@ __global__ void k_init_v(const vec<V>* __restrict__ r, const vec<V>* __restrict_
   }
 }
 
+// Streaming kernels below unroll U elements per thread and step (loads of
+// all U first, then arithmetic, then stores) so each wave keeps U times
+// more bytes in flight; element order inside a thread is fixed, so the
+// reductions stay deterministic.
+#define UNROLLED_LOOP(U, q, stride, n)                                       \
+  const long long stride = (long long)gridDim.x * blockDim.x;                 \
+  long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;             \
+  for (; q + (U - 1) * stride < (n); q += (U) * stride)
+
 // A: p = M^-1 v - p k1 (:306-320)
-template <typename V>
+template <typename V, int U = 4>
 __global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
                           vec<V>* __restrict__ p, long long n, const QmrState* st) {
   if (st->done) return;
   const c128 k1 = st->k1;
-  GRID_LOOP(q, n) p[q] = dn<V>(up(v[q]) * up(invd[q]) - up(p[q]) * k1);
+  UNROLLED_LOOP(U, q, stride, n) {
+    vec<V> a[U], b[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = v[q + u * stride];
+      b[u] = invd[q + u * stride];
+      c[u] = p[q + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) p[q + u * stride] = dn<V>(up(a[u]) * up(b[u]) - up(c[u]) * k1);
+  }
+  for (; q < n; q += stride) p[q] = dn<V>(up(v[q]) * up(invd[q]) - up(p[q]) * k1);
 }
 
 // C: rho' = ||t - beta v|| and the step-4 scalars (:332-345)
 template <typename V>
 __global__ void k_vtnorm(const vec<V>* __restrict__ t, const vec<V>* __restrict__ v, long long n,
-                         QmrState* st, double* part) {
+                         const QmrState* st, double* part) {
   if (st->done) return;
   const c128 beta = st->beta;
-  double a[1] = {0.0}, tot[1];
-  GRID_LOOP(q, n) a[0] += abs2(up(dn<V>(up(t[q]) - up(v[q]) * beta)));
+  double a[1] = {0.0};
+  UNROLLED_LOOP(4, q, stride, n) {
+    vec<V> tt[4], vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      tt[u] = t[q + u * stride];
+      vv[u] = v[q + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[0] += abs2(up(dn<V>(up(tt[u]) - up(vv[u]) * beta)));
+  }
+  for (; q < n; q += stride) a[0] += abs2(up(dn<V>(up(t[q]) - up(v[q]) * beta)));
   block_sum<1>(a);
-  if (grid_finish<1>(a, part, &st->ticket, tot) && threadIdx.x == 0) {
+  block_partial<1>(a, part);
+}
+__global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
+  if (st->done) return;
+  double tot[1];
+  finalize_sum<1>(part, nb, tot);
+  if (threadIdx.x == 0) {
+    const c128 beta = st->beta;
     const double rho = sqrt(tot[0]);
     st->rho_np = rho;
     const double bb = beta.re * beta.re + beta.im * beta.im;
@@ -144,32 +196,50 @@ template <typename V>
 __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x,
                          const vec<V>* __restrict__ t, vec<V>* __restrict__ s, vec<V>* __restrict__ r,
                          vec<V>* __restrict__ v, const vec<V>* __restrict__ invd, long long n,
-                         QmrState* st, double* part, double* hist) {
+                         const QmrState* st, double* part) {
   if (st->done) return;
   const c128 eta = st->eta, beta = st->beta;
   const double k2 = st->k2s, rho = st->rho_np;
-  double a[3] = {0.0, 0.0, 0.0}, tot[3];
-  GRID_LOOP(q, n) {
-    const c128 tq = up(t[q]);
-    const c128 dd = up(p[q]) * eta + up(d[q]) * k2;
-    const vec<V> ddv = dn<V>(dd);
+  double a[3] = {0.0, 0.0, 0.0};
+  auto body = [&](long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_, vec<V> sq, vec<V> rq,
+                  vec<V> vq, vec<V> iq) {
+    const c128 tq = up(tq_);
+    const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
     d[q] = ddv;
-    x[q] = dn<V>(up(x[q]) + up(ddv));
-    const vec<V> ssv = dn<V>(tq * eta + up(s[q]) * k2);
+    x[q] = dn<V>(up(xq) + up(ddv));
+    const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
     s[q] = ssv;
-    const vec<V> rrv = dn<V>(up(r[q]) - up(ssv));
+    const vec<V> rrv = dn<V>(up(rq) - up(ssv));
     r[q] = rrv;
-    const vec<V> vt = dn<V>(tq - up(v[q]) * beta);
+    const vec<V> vt = dn<V>(tq - up(vq) * beta);
     const vec<V> vv = dn<V>(divr(up(vt), rho));
     v[q] = vv;
     a[0] += abs2(up(rrv));
     const c128 vd = up(vv);
-    const c128 pr = vd * (vd * up(invd[q]));
+    const c128 pr = vd * (vd * up(iq));
     a[1] += pr.re;
     a[2] += pr.im;
+  };
+  UNROLLED_LOOP(2, q, stride, n) {
+    vec<V> P_[2], D_[2], X_[2], T_[2], S_[2], R_[2], Vv[2], I_[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const long long e = q + u * stride;
+      P_[u] = p[e]; D_[u] = d[e]; X_[u] = x[e]; T_[u] = t[e];
+      S_[u] = s[e]; R_[u] = r[e]; Vv[u] = v[e]; I_[u] = invd[e];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) body(q + u * stride, P_[u], D_[u], X_[u], T_[u], S_[u], R_[u], Vv[u], I_[u]);
   }
+  for (; q < n; q += stride) body(q, p[q], d[q], x[q], t[q], s[q], r[q], v[q], invd[q]);
   block_sum<3>(a);
-  if (grid_finish<3>(a, part, &st->ticket, tot) && threadIdx.x == 0) {
+  block_partial<3>(a, part);
+}
+__global__ void k_fin_update(const double* part, int nb, QmrState* st, double* hist) {
+  if (st->done) return;
+  double tot[3];
+  finalize_sum<3>(part, nb, tot);
+  if (threadIdx.x == 0) {
     const double rnorm = sqrt(tot[0]);
     st->rnorm = rnorm;
     const int it = st->iter;
@@ -208,16 +278,46 @@ __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, v
 // ||b - t||^2 / ||b||^2 for the true residual (t = K x)
 template <typename V>
 __global__ void k_resnorm(const vec<V>* __restrict__ b, const vec<V>* __restrict__ t, long long n,
-                          double* out, double* part, unsigned* ticket) {
-  double a[2] = {0.0, 0.0}, tot[2];
+                          double* part) {
+  double a[2] = {0.0, 0.0};
   GRID_LOOP(q, n) {
     const c128 bq = up(b[q]);
     a[0] += abs2(bq - up(t[q]));
     a[1] += abs2(bq);
   }
   block_sum<2>(a);
-  if (grid_finish<2>(a, part, ticket, tot) && threadIdx.x == 0) {
+  block_partial<2>(a, part);
+}
+__global__ void k_fin_resnorm(const double* part, int nb, double* out) {
+  double tot[2];
+  finalize_sum<2>(part, nb, tot);
+  if (threadIdx.x == 0) {
     out[0] = sqrt(tot[0]);
     out[1] = sqrt(tot[1]);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: outer iterative refinement in complex128 around an inner
+// complex64 QMR.  r = b - K x is formed in FP64; the inner right-hand side is
+// r / ||r|| rounded to complex64 (unit norm keeps every FP32 intermediate far
+// from underflow); the inner solution e is added back as x += ||r|| e in FP64.
+
+// inner rhs: bi = (b - t) * (1/rn) in complex64, rn = out[0] of k_resnorm
+__global__ void k_inner_rhs(const c128* __restrict__ b, const c128* __restrict__ t,
+                            const double* __restrict__ rn, c64* __restrict__ bi, long long n) {
+  const double s = 1.0 / rn[0];
+  GRID_LOOP(q, n) bi[q] = dn<float>((b[q] - t[q]) * s);
+}
+
+// x += rn * e  (e complex64, x complex128)
+__global__ void k_refine(c128* __restrict__ x, const c64* __restrict__ e,
+                         const double* __restrict__ rn, long long n) {
+  const double s = rn[0];
+  GRID_LOOP(q, n) x[q] = x[q] + up(e[q]) * s;
+}
+
+template <typename To, typename From>
+__global__ void k_cast(const cplx<From>* __restrict__ a, cplx<To>* __restrict__ b, long long n) {
+  GRID_LOOP(q, n) b[q] = cplx<To>{(To)a[q].re, (To)a[q].im};
 }

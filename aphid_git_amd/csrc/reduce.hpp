@@ -1,12 +1,13 @@
 // reduce.hpp -- wave64 / block / grid reductions with FP64 accumulation and
 // the device-resident QMR scalar state.
 //
-// Grid reductions use the last-block-finalises pattern: every block writes
-// its partial sums to part[block], publishes them with an agent-scope fence
-// and takes a ticket; the block holding the last ticket sums the partials in
-// a fixed order (deterministic, independent of scheduling), applies the
-// scalar recurrence and resets the ticket.  No second launch, no host
-// round-trip.
+// Grid reductions are two launches: every block of the streaming kernel
+// writes its partial sums to part[block] (plain stores), and a one-block
+// finalize kernel sums them in a fixed order (deterministic, independent of
+// scheduling) and applies the scalar recurrence on the device.  The kernel
+// boundary makes the partials visible; an in-kernel last-block scheme would
+// need an agent-scope release fence per block, which on this GPU writes back
+// the L2 and costs more than the extra launch.
 #pragma once
 #include "cplx.hpp"
 
@@ -42,50 +43,26 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   }
 }
 
-__device__ __forceinline__ double ld_agent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Publish this block's NV partials (valid in thread 0) to part[block].
+template <int NV>
+__device__ __forceinline__ void block_partial(const double (&v)[NV], double* part) {
+  if (threadIdx.x == 0) {
+    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[(size_t)bid * NV + q] = v[q];
+  }
 }
 
-// Publish this block's NV partials (valid in thread 0) and return true in
-// every thread of the block that finishes last; that block then holds the
-// fixed-order grid totals in tot[] (all threads).
+// In a one-block finalize kernel: the fixed-order total of nb partials
+// (thread t sums blocks t, t+T, ...; then a block tree), valid in thread 0.
 template <int NV>
-__device__ __forceinline__ bool grid_finish(double (&v)[NV], double* part, unsigned* ticket,
-                                            double (&tot)[NV]) {
-  __shared__ int am_last;
-  __shared__ double sh_tot[NV];
-  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ void finalize_sum(const double* __restrict__ part, int nb, double (&tot)[NV]) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) st_agent(&part[(size_t)bid * NV + q], v[q]);
-    __threadfence();
-    unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (t == nb - 1);
-  }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-  // fixed-order two-level sum: thread t sums blocks t, t+T, ...; then block_sum
-  double s[NV];
+  for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
 #pragma unroll
-  for (int q = 0; q < NV; ++q) s[q] = 0.0;
-  for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-#pragma unroll
-    for (int q = 0; q < NV; ++q) s[q] += ld_agent(&part[(size_t)b * NV + q]);
-  block_sum<NV>(s);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) sh_tot[q] = s[q];
-    *ticket = 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) tot[q] = sh_tot[q];
-  return true;
+    for (int q = 0; q < NV; ++q) tot[q] += part[(size_t)b * NV + q];
+  block_sum<NV>(tot);
 }
 
 // Scalars of the coupled two-term recurrence (src/aphid_qmr.f90:63-65),
@@ -99,7 +76,6 @@ struct QmrState {
   double rnorm, rhsnorm, target;
   int iter, max_it, status, done;
   int hist_cap, pad0;
-  unsigned ticket, pad1;
 };
 
 enum : int { ST_CONV = 0, ST_MAXIT = 1, ST_BREAK = 2 };

@@ -50,6 +50,17 @@ struct ApplyEpilogue {
   double* part;     // per-block partials (2 doubles per block)
 };
 
+// 32-bit byte offsets from a base pointer: lets the compiler address every
+// array as SGPR base + one 32-bit VGPR offset (all buffers are < 4 GiB)
+template <typename T>
+__device__ __forceinline__ T ldo(const T* __restrict__ base, unsigned boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* __restrict__ base, unsigned boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
 template <typename R, int TK, int TJ>
 __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* __restrict__ in,
                                                   cplx<R>* __restrict__ out, int chunk,
@@ -57,29 +68,55 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   using C = cplx<R>;
   constexpr int LK = TK + 2, LJ = TJ + 2, LS = LK * LJ;
   constexpr int NH = 2 * TK + 2 * TJ;
+  constexpr unsigned CS = sizeof(C);
   static_assert(NH <= TK * TJ, "halo larger than the block");
   __shared__ C sm[7][LS];  // Ax, Ay, Az, Phi, k2y, k2z, k2p of plane i
+  // per-tile 1-D tables: [0]nP [1]nM [2]eP [3]eM [4]dN [5]dE [6]rdN [7]rdE
+  __shared__ R tyj[8][TJ];
+  __shared__ R tzk[8][TK];
 
   if (ep.st && ep.st->done) return;
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
-  const long long nn = op.nn, pl = (long long)ny * nz;
+  const unsigned pl = (unsigned)ny * nz;
+  const unsigned nnb = (unsigned)op.nn * CS;  // component stride in bytes
   const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
   const int k = blockIdx.x * TK + tk, j = blockIdx.y * TJ + tj;
   const int i0 = blockIdx.z * chunk;
   const int i1 = min(nx, i0 + chunk);
   const bool own = (j < ny) && (k < nz);
-  const long long col = own ? (long long)j * nz + k : 0;
-
-  const C* __restrict__ src[4] = {in, in + nn, in + 2 * nn, in + 3 * nn};
-  const C* __restrict__ K2x = op.k2;
-  const C* __restrict__ K2y = op.k2 + nn;
-  const C* __restrict__ K2z = op.k2 + 2 * nn;
-  const C* __restrict__ K2p = op.k2 + 3 * nn;
+  const unsigned colb = own ? ((unsigned)j * nz + k) * CS : 0u;
+  const unsigned plb = pl * CS;
   const C zero = {R(0), R(0)};
 
-  auto ld = [&](const C* __restrict__ a, int i) -> C {
-    return (own && i >= 0 && i < nx) ? a[(long long)i * pl + col] : zero;
+  if (threadIdx.x < TJ) {
+    const int jj = blockIdx.y * TJ + threadIdx.x;
+    const bool v = jj < ny;
+    tyj[0][threadIdx.x] = v ? op.nP[1][jj] : R(0);
+    tyj[1][threadIdx.x] = v ? op.nM[1][jj] : R(0);
+    tyj[2][threadIdx.x] = v ? op.eP[1][jj] : R(0);
+    tyj[3][threadIdx.x] = v ? op.eM[1][jj] : R(0);
+    tyj[4][threadIdx.x] = v ? op.dN[1][jj] : R(0);
+    tyj[5][threadIdx.x] = v ? op.dE[1][jj] : R(0);
+    tyj[6][threadIdx.x] = v ? op.rdN[1][jj] : R(0);
+    tyj[7][threadIdx.x] = v ? op.rdE[1][jj] : R(0);
+  }
+  if (threadIdx.x < TK) {
+    const int kk = blockIdx.x * TK + threadIdx.x;
+    const bool v = kk < nz;
+    tzk[0][threadIdx.x] = v ? op.nP[2][kk] : R(0);
+    tzk[1][threadIdx.x] = v ? op.nM[2][kk] : R(0);
+    tzk[2][threadIdx.x] = v ? op.eP[2][kk] : R(0);
+    tzk[3][threadIdx.x] = v ? op.eM[2][kk] : R(0);
+    tzk[4][threadIdx.x] = v ? op.dN[2][kk] : R(0);
+    tzk[5][threadIdx.x] = v ? op.dE[2][kk] : R(0);
+    tzk[6][threadIdx.x] = v ? op.rdN[2][kk] : R(0);
+    tzk[7][threadIdx.x] = v ? op.rdE[2][kk] : R(0);
+  }
+
+  // own value of component c (0..3) or k2 grid c (0..3) at plane i
+  auto ldc = [&](const C* __restrict__ base, int c, int i) -> C {
+    return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb) : zero;
   };
 
   // halo cell of this thread (if any): rows j=-1 and j=TJ over the tile's
@@ -87,7 +124,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   int hj = 0, hk = 0;
   const bool hv = threadIdx.x < NH;
   if (hv) {
-    int h = threadIdx.x;
+    const int h = threadIdx.x;
     if (h < TK) { hj = -1; hk = h; }
     else if (h < 2 * TK) { hj = TJ; hk = h - TK; }
     else if (h < 2 * TK + TJ) { hj = h - 2 * TK; hk = -1; }
@@ -95,54 +132,48 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   }
   const int gj = blockIdx.y * TJ + hj, gk = blockIdx.x * TK + hk;
   const bool hin = hv && gj >= 0 && gj < ny && gk >= 0 && gk < nz;
-  const long long hcol = hin ? (long long)gj * nz + gk : 0;
+  const unsigned hcolb = hin ? ((unsigned)gj * nz + gk) * CS : 0u;
   const int hl = (hj + 1) * LK + (hk + 1);
   const int ol = (tj + 1) * LK + (tk + 1);
-
-  // per-thread (j,k) tables
-  const R nPy = own ? op.nP[1][j] : R(0), nMy = own ? op.nM[1][j] : R(0);
-  const R nPz = own ? op.nP[2][k] : R(0), nMz = own ? op.nM[2][k] : R(0);
-  const R dNy = own ? op.dN[1][j] : R(0), dNz = own ? op.dN[2][k] : R(0);
-  const bool yrow = own && j < ny - 1, zrow = own && k < nz - 1;
-  const bool pjk = own && j > 0 && j < ny - 1 && k > 0 && k < nz - 1;
 
   C pa[4], ca[4], na[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    pa[c] = ld(src[c], i0 - 1);
-    ca[c] = ld(src[c], i0);
+    pa[c] = ldc(in, c, i0 - 1);
+    ca[c] = ldc(in, c, i0);
   }
-  C k2x_p = ld(K2x, i0 - 1), k2x_c = ld(K2x, i0);
-  C k2y_c = ld(K2y, i0), k2z_c = ld(K2z, i0), k2p_c = ld(K2p, i0);
+  const C* __restrict__ K2 = op.k2;
+  C k2x_p = ldc(K2, 0, i0 - 1), k2x_c = ldc(K2, 0, i0), k2p_c = ldc(K2, 3, i0);
 
   double eacc_re = 0.0, eacc_im = 0.0;
 
   for (int i = i0; i < i1; ++i) {
-    // plane i+1, own column (x-neighbours of the next step)
+    // plane i+1 of the own column: the x-neighbours of this step
 #pragma unroll
-    for (int c = 0; c < 4; ++c) na[c] = ld(src[c], i + 1);
-    const C k2x_n = ld(K2x, i + 1), k2y_n = ld(K2y, i + 1), k2z_n = ld(K2z, i + 1);
-    const C k2p_n = ld(K2p, i + 1);
+    for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
+    const C k2x_n = ldc(K2, 0, i + 1), k2p_n = ldc(K2, 3, i + 1);
 
-    // stage plane i
+    // stage plane i: own values, then the halo
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
-    sm[4][ol] = k2y_c;
-    sm[5][ol] = k2z_c;
+    sm[4][ol] = ldc(K2, 1, i);
+    sm[5][ol] = ldc(K2, 2, i);
     sm[6][ol] = k2p_c;
     if (hv) {
-      const long long g = (long long)i * pl + hcol;
+      const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? src[c][g] : zero;
-      sm[4][hl] = hin ? K2y[g] : zero;
-      sm[5][hl] = hin ? K2z[g] : zero;
-      sm[6][hl] = hin ? K2p[g] : zero;
+      for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : zero;
+#pragma unroll
+      for (int c = 1; c < 4; ++c) sm[3 + c][hl] = hin ? ldo(K2, c * nnb + g) : zero;
     }
     __syncthreads();
 
     if (own) {
-      const long long o = (long long)i * pl + col;
+      const unsigned ob = (unsigned)i * plb + colb;
       const R nPx = op.nP[0][i], nMx = op.nM[0][i], dNx = op.dN[0][i];
+      const R nPy = tyj[0][tj], nMy = tyj[1][tj], nPz = tzk[0][tk], nMz = tzk[1][tk];
+      const R dNy = tyj[4][tj], dNz = tzk[4][tk];
+      const C k2y_c = sm[4][ol], k2z_c = sm[5][ol];
       C rx = zero, ry = zero, rz = zero, rp = zero;
       // ---- x-edge row, inprod_x (src/aphid.f90:841-866)
       if (i < nx - 1) {
@@ -160,9 +191,10 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
         const R vol = op.dE[0][i] * dNy * dNz;
         rx = (acc + cpl * op.rdE[0][i]) * vol;
       }
+      sto(out, ob, rx);
       // ---- y-edge row, inprod_y (src/aphid.f90:970-995)
-      if (yrow) {
-        const R cN = op.eP[1][j], cS = op.eM[1][j];
+      if (j < ny - 1) {
+        const R cN = tyj[2][tj], cS = tyj[3][tj];
         const R sm6 = cN + cS + nPx + nMx + nPz + nMz;
         const C c1 = {k2y_c.re - sm6, k2y_c.im};
         C acc = c1 * ca[1];
@@ -173,12 +205,13 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
         acc += nPz * sm[1][ol + 1];
         acc += nMz * sm[1][ol - 1];
         const C cpl = sm[3][ol + LK] * (sm[6][ol + LK] - k2y_c) - ca[3] * (k2p_c - k2y_c);
-        const R vol = dNx * op.dE[1][j] * dNz;
-        ry = (acc + cpl * op.rdE[1][j]) * vol;
+        const R vol = dNx * tyj[5][tj] * dNz;
+        ry = (acc + cpl * tyj[7][tj]) * vol;
       }
+      sto(out, nnb + ob, ry);
       // ---- z-edge row, inprod_z (src/aphid.f90:1098-1123)
-      if (zrow) {
-        const R cU = op.eP[2][k], cD = op.eM[2][k];
+      if (k < nz - 1) {
+        const R cU = tzk[2][tk], cD = tzk[3][tk];
         const R sm6 = cU + cD + nPx + nMx + nPy + nMy;
         const C c1 = {k2z_c.re - sm6, k2z_c.im};
         C acc = c1 * ca[2];
@@ -189,11 +222,12 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
         acc += nPy * sm[2][ol + LK];
         acc += nMy * sm[2][ol - LK];
         const C cpl = sm[3][ol + 1] * (sm[6][ol + 1] - k2z_c) - ca[3] * (k2p_c - k2z_c);
-        const R vol = dNx * dNy * op.dE[2][k];
-        rz = (acc + cpl * op.rdE[2][k]) * vol;
+        const R vol = dNx * dNy * tzk[5][tk];
+        rz = (acc + cpl * tzk[7][tk]) * vol;
       }
+      sto(out, 2 * nnb + ob, rz);
       // ---- interior node row, inprod_p (src/aphid.f90:1229-1278)
-      if (pjk && i > 0 && i < nx - 1) {
+      if (j > 0 && j < ny - 1 && k > 0 && k < nz - 1 && i > 0 && i < nx - 1) {
         const C s1 = k2p_c, s2 = k2x_c, s3 = k2x_p;
         const C s4 = k2y_c, s5 = sm[4][ol - LK];
         const C s6 = k2z_c, s7 = sm[5][ol - 1];
@@ -207,7 +241,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
         acc += c5 * sm[3][ol - LK];
         acc += c6 * sm[3][ol + 1];
         acc += c7 * sm[3][ol - 1];
-        const R rx_ = op.rdN[0][i], ry_ = op.rdN[1][j], rz_ = op.rdN[2][k];
+        const R rx_ = op.rdN[0][i], ry_ = tyj[6][tj], rz_ = tzk[6][tk];
         C cp = ((s2 - s1) * rx_) * ca[0];
         cp += (-((s3 - s1) * rx_)) * pa[0];
         cp += ((s4 - s1) * ry_) * ca[1];
@@ -217,10 +251,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
         const R vol = dNx * dNy * dNz;
         rp = acc * vol + cp * vol;
       }
-      out[o] = rx;
-      out[o + nn] = ry;
-      out[o + 2 * nn] = rz;
-      out[o + 3 * nn] = rp;
+      sto(out, 3 * nnb + ob, rp);
       if (ep.st) {  // eps partial: sum_c in_c * out_c in FP64
         const cplx<double> a0 = ccast<double>(ca[0]) * ccast<double>(rx);
         const cplx<double> a1 = ccast<double>(ca[1]) * ccast<double>(ry);
@@ -238,22 +269,27 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     }
     k2x_p = k2x_c;
     k2x_c = k2x_n;
-    k2y_c = k2y_n;
-    k2z_c = k2z_n;
     k2p_c = k2p_n;
   }
 
   if (ep.st) {
-    double v[2] = {eacc_re, eacc_im}, tot[2];
+    double v[2] = {eacc_re, eacc_im};
     block_sum<2>(v);
-    if (grid_finish<2>(v, ep.part, &ep.st->ticket, tot) && threadIdx.x == 0) {
-      QmrState* st = ep.st;
-      st->eps = {tot[0], tot[1]};
-      st->beta = cdiv(st->eps, st->del);  // src/aphid_qmr.f90:330
-      if (is_zero(st->eps) || !finite_c(st->beta)) {
-        st->status = ST_BREAK;
-        st->done = 1;
-      }
+    block_partial<2>(v, ep.part);
+  }
+}
+
+// eps = p^T K p (the partials of k_apply), beta = eps/del (src/aphid_qmr.f90:329-330)
+__global__ void k_fin_eps(const double* part, int nb, QmrState* st) {
+  if (st->done) return;
+  double tot[2];
+  finalize_sum<2>(part, nb, tot);
+  if (threadIdx.x == 0) {
+    st->eps = {tot[0], tot[1]};
+    st->beta = cdiv(st->eps, st->del);
+    if (is_zero(st->eps) || !finite_c(st->beta)) {
+      st->status = ST_BREAK;
+      st->done = 1;
     }
   }
 }

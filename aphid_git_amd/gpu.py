@@ -181,6 +181,14 @@ class Solver:
                "aphid_apply")
         return out
 
+    def apply_prec(self, u, precision: int):
+        """K u with the complex128 (0) or complex64 (1) stencil."""
+        u = [_c128(a, s) for a, s in zip(u, comp_shapes(*self.shape))]
+        out = self._zeros4()
+        _check(lib().aphid_apply_prec(self._ctx, precision, *[_p(a) for a in u],
+                                      *[_p(a) for a in out]), "aphid_apply_prec")
+        return out
+
     def build_rhs(self, mode: int, sources: Sequence[Tuple[int, int, int, int]] = ()):
         out = self._zeros4()
         ns = len(sources)
@@ -261,6 +269,6 @@ class Solver:
 
 EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_set_mesh",
            "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
-           "aphid_get_diag", "aphid_apply", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
+           "aphid_get_diag", "aphid_apply", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
            "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
            "aphid_fetch_solution", "aphid_time_iteration"]

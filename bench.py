@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-8
+MAX_IT = 100000
 
 
 def log(msg):
@@ -94,7 +95,7 @@ def cpu_baseline(case, gpu_iters: float, sample_its: int = 30):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-its", type=int, default=30)
@@ -115,14 +116,14 @@ def main():
     b = s.setup_case(case)
     s.load_rhs(b)
     for w in range(args.warmup):
-        r = s.solve_resident(TOL, 20000)
+        r = s.solve_resident(TOL, MAX_IT)
         log(f"[bench] warmup {w}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
             f"{r.seconds_qmr:.3f}s")
 
     results = []
     t0 = time.perf_counter()  # each solve returns after its stream has drained
     for k in range(args.steps):
-        r = s.solve_resident(TOL, 20000)
+        r = s.solve_resident(TOL, MAX_IT)
         results.append(r)
         log(f"[bench] step {k}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
             f"{r.seconds_qmr:.3f}s")

@@ -44,8 +44,12 @@ extern "C" {
 /* aphid_qmr_solve flags */
 #define APHID_F_X0 1u          /* x0 arrays hold an initial guess (else x0 = 0) */
 #define APHID_F_NO_TRUE_RES 2u /* skip the final ||b - A x|| check */
-#define APHID_F_MIXED 4u       /* complex64 Krylov vectors + FP64 scalars and
-                                  FP64 iterative refinement (see report) */
+#define APHID_F_MIXED 4u       /* complex64 stencil and Krylov vectors, FP64
+                                  inner products and recurrence scalars, inside
+                                  an FP64 iterative refinement on the true
+                                  residual b - K x; iters counts inner
+                                  iterations, outer_iters the refinement sweeps,
+                                  hist the true relative residual per sweep */
 
 typedef struct aphid_ctx aphid_ctx;
 
@@ -107,6 +111,14 @@ int aphid_get_diag(aphid_ctx *ctx, double *dgx_c, double *dgy_c, double *dgz_c,
 int aphid_apply(aphid_ctx *ctx, const double *in_x, const double *in_y,
                 const double *in_z, const double *in_p, double *out_x,
                 double *out_y, double *out_z, double *out_p);
+
+/* aphid_apply with the stencil in a chosen precision: 0 = complex128
+ * (identical to aphid_apply), 1 = complex64 operator data and arithmetic
+ * (the mixed-precision inner operator; inputs and outputs stay
+ * complex128 on the host). */
+int aphid_apply_prec(aphid_ctx *ctx, int precision, const double *in_x,
+                     const double *in_y, const double *in_z, const double *in_p,
+                     double *out_x, double *out_y, double *out_z, double *out_p);
 
 /* Right-hand side (gen_rhs, src/aphid.f90:534-743) into caller arrays.
  * Mode 5 takes nsrc dipoles with dir in {+1,-1,+2,-2,+3,-3} = +X..-Z and a
