@@ -40,16 +40,32 @@ constexpr int kMaxPartBlocks = 1 << 16; // capacity of the partial-sum buffer
 // ---------------------------------------------------------------------------
 // stencil launch shapes (compile-time tiles, runtime chunk)
 struct Tile {
-  int tk, tj, chunk;
+  int tk, tj, chunk;  // chunk = x-planes per block (march); 0 = direct variant
 };
 const Tile kTiles[] = {{16, 16, 0}, {32, 8, 0}, {64, 4, 0}, {16, 8, 0}, {32, 4, 0}};
 
 template <typename R>
 hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, cplx<R>* out,
                         ApplyEpilogue ep, hipStream_t s) {
-  const int chunk = std::max(1, t.chunk);
-  dim3 grid((op.nz + t.tk - 1) / t.tk, (op.ny + t.tj - 1) / t.tj, (op.nx + chunk - 1) / chunk);
+  const int ntk = (op.nz + t.tk - 1) / t.tk, ntj = (op.ny + t.tj - 1) / t.tj;
   dim3 block(t.tk * t.tj);
+  if (t.chunk <= 0) {
+    dim3 grid((unsigned)ntk * ntj * op.nx);
+#define APHID_TILE(TK, TJ)                                                                \
+    if (t.tk == TK && t.tj == TJ) {                                                       \
+      hipLaunchKernelGGL((k_apply_direct<R, TK, TJ>), grid, block, 0, s, op, in, out, ep); \
+      return hipGetLastError();                                                           \
+    }
+    APHID_TILE(16, 16)
+    APHID_TILE(32, 8)
+    APHID_TILE(64, 4)
+    APHID_TILE(16, 8)
+    APHID_TILE(32, 4)
+#undef APHID_TILE
+    return hipErrorInvalidValue;
+  }
+  const int chunk = t.chunk;
+  dim3 grid((unsigned)ntk * ntj * ((op.nx + chunk - 1) / chunk));
 #define APHID_TILE(TK, TJ)                                                           \
   if (t.tk == TK && t.tj == TJ) {                                                    \
     hipLaunchKernelGGL((k_apply<R, TK, TJ>), grid, block, 0, s, op, in, out, chunk, ep); \
@@ -65,8 +81,9 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
 }
 
 unsigned apply_blocks(const Tile& t, int nx, int ny, int nz) {
-  const int chunk = std::max(1, t.chunk);
-  return (unsigned)((nz + t.tk - 1) / t.tk) * ((ny + t.tj - 1) / t.tj) * ((nx + chunk - 1) / chunk);
+  const unsigned ntk = (nz + t.tk - 1) / t.tk, ntj = (ny + t.tj - 1) / t.tj;
+  if (t.chunk <= 0) return ntk * ntj * (unsigned)nx;
+  return ntk * ntj * (unsigned)((nx + t.chunk - 1) / t.chunk);
 }
 
 // 1-D tables of one axis (see OpArgs in stencil.hpp), from node coordinates
@@ -303,7 +320,7 @@ int autotune(aphid_ctx* c, int prec) {
   }
   int rc = ensure_vectors(c, prec == 1);
   if (rc) return rc;
-  const int chunks[] = {4, 8, 16, 32};
+  const int chunks[] = {0, 2, 4, 6, 8, 12, 16};  // 0 = direct variant
   float best = 1e30f;
   Tile bt = c->tile[prec];
   const int reps = c->nn > 2000000 ? 5 : 20;
@@ -397,9 +414,34 @@ int graph_iters(aphid_ctx* c, int prec) {
   return std::max(4, std::min(128, (int)std::ceil(3e-3 / t_it)));
 }
 
+// APHID_NO_GRAPH=1 enqueues the iterations directly on the stream instead
+// of replaying a captured graph (same kernels and launch shapes); profilers
+// that intercept hipGraphLaunch need it.
+bool use_graphs() {
+  const char* e = getenv("APHID_NO_GRAPH");
+  return !(e && e[0] == '1');
+}
+
 template <typename V>
 int run_loop(aphid_ctx* c) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
+  if (!use_graphs()) {
+    const int k = graph_iters(c, prec);
+    const long long max_launch = (long long)c->hst->max_it / k + 3;
+    int last_iter = c->hst->iter;
+    for (long long l = 0;; ++l) {
+      for (int q = 0; q < k; ++q) {
+        const int rc = enqueue_iteration<V>(c);
+        if (rc) return rc;
+      }
+      CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
+      CK(hipStreamSynchronize(c->s));
+      if (c->hst->done) return 0;
+      if (c->hst->iter <= last_iter || l > max_launch)
+        return fail(APHID_E_STATE, "QMR state did not advance on the device");
+      last_iter = c->hst->iter;
+    }
+  }
   if (!c->gexec[prec]) {
     const int k = graph_iters(c, prec);
     hipGraph_t g;
@@ -1076,7 +1118,7 @@ int aphid_get_tile(aphid_ctx* c, int precision, int* tk, int* tj, int* chunk) {
 }
 
 int aphid_set_tile(aphid_ctx* c, int precision, int tk, int tj, int chunk) {
-  if (!c || precision < 0 || precision > 1 || chunk < 1) return fail(APHID_E_ARG, "bad argument");
+  if (!c || precision < 0 || precision > 1 || chunk < 0) return fail(APHID_E_ARG, "bad argument");
   bool ok = false;
   for (const Tile& t : kTiles) ok |= (t.tk == tk && t.tj == tj);
   if (!ok) return fail(APHID_E_ARG, "unsupported tile");
@@ -1174,6 +1216,37 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   CK(hipMemcpy(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost));
   if (c->hst->done) return fail(APHID_E_STATE, "Krylov process broke down while timing");
   return 0;
+}
+
+int aphid_time_copy(aphid_ctx* c, double bytes, int reps, double* seconds) {
+  if (!c || !seconds || reps < 1 || !(bytes >= 16.0) || bytes > 64.0 * (1ull << 30))
+    return fail(APHID_E_ARG, "bad argument");
+  CK(hipSetDevice(c->dev));
+  const long long n = (long long)(bytes / sizeof(c128));
+  c128 *a = nullptr, *b = nullptr;
+  CK(hipMalloc(&a, sizeof(c128) * n));
+  hipError_t e = hipMalloc(&b, sizeof(c128) * n);
+  if (e != hipSuccess) {
+    hipFree(a);
+    return fail(APHID_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  int rc = 0;
+  float ms = 0.f;
+  do {
+    if ((e = hipMemsetAsync(a, 0, sizeof(c128) * n, c->s)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_copy, dim3(16384), dim3(kBlock), 0, c->s, a, b, n);  // warm
+    if ((e = hipEventRecord(c->ev0, c->s)) != hipSuccess) break;
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy, dim3(16384), dim3(kBlock), 0, c->s, a, b, n);
+    if ((e = hipEventRecord(c->ev1, c->s)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) break;
+    e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  } while (0);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) rc = fail(APHID_E_HIP, std::string("copy probe: ") + hipGetErrorString(e));
+  *seconds = 1e-3 * ms / reps;
+  return rc;
 }
 
 }  // extern "C"

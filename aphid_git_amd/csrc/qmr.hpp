@@ -321,3 +321,8 @@ template <typename To, typename From>
 __global__ void k_cast(const cplx<From>* __restrict__ a, cplx<To>* __restrict__ b, long long n) {
   GRID_LOOP(q, n) b[q] = cplx<To>{(To)a[q].re, (To)a[q].im};
 }
+
+// plain copy (bandwidth probe)
+__global__ void k_copy(const c128* __restrict__ a, c128* __restrict__ b, long long n) {
+  GRID_LOOP(q, n) b[q] = a[q];
+}

@@ -61,6 +61,160 @@ __device__ __forceinline__ void sto(T* __restrict__ base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
+// 1-D table values of one axis at one coordinate
+template <typename R>
+struct Ax1 {
+  R nP, nM, eP, eM, dN, dE, rdN, rdE;
+};
+template <typename R>
+__device__ __forceinline__ Ax1<R> axis_at(const OpArgs<R>& op, int a, int q) {
+  return {op.nP[a][q], op.nM[a][q], op.eP[a][q], op.eM[a][q],
+          op.dN[a][q], op.dE[a][q], op.rdN[a][q], op.rdE[a][q]};
+}
+
+// The four rows of K u at node (i,j,k).  `a` gives field values
+// a.fv(c, d) (c: 0 Ax, 1 Ay, 2 Az, 3 Phi) and k^2 values a.kv(c, d)
+// (c: 0 k2x, 1 k2y, 2 k2z, 3 k2p) at direction d: 0 centre, 1 i+1, 2 i-1,
+// 3 j+1, 4 j-1, 5 k+1, 6 k-1, with 0 outside the domain.  Shared by both
+// kernels so that they perform identical arithmetic.
+template <typename R, typename A>
+__device__ __forceinline__ void rows4(int i, int j, int k, int nx, int ny, int nz, const Ax1<R>& X,
+                                      const Ax1<R>& Y, const Ax1<R>& Z, A& a, cplx<R>& rx,
+                                      cplx<R>& ry, cplx<R>& rz, cplx<R>& rp) {
+  using C = cplx<R>;
+  const C zero = {R(0), R(0)};
+  rx = ry = rz = rp = zero;
+  const C k2p_c = a.kv(3, 0);
+  // ---- x-edge row, inprod_x (src/aphid.f90:841-866)
+  if (i < nx - 1) {
+    const C k2x_c = a.kv(0, 0);
+    const R sm6 = X.eP + X.eM + Y.nP + Y.nM + Z.nP + Z.nM;
+    const C c1 = {k2x_c.re - sm6, k2x_c.im};
+    C acc = c1 * a.fv(0, 0);
+    acc += X.eP * a.fv(0, 1);
+    acc += X.eM * a.fv(0, 2);
+    acc += Y.nP * a.fv(0, 3);
+    acc += Y.nM * a.fv(0, 4);
+    acc += Z.nP * a.fv(0, 5);
+    acc += Z.nM * a.fv(0, 6);
+    const C cpl = a.fv(3, 1) * (a.kv(3, 1) - k2x_c) - a.fv(3, 0) * (k2p_c - k2x_c);
+    const R vol = X.dE * Y.dN * Z.dN;
+    rx = (acc + cpl * X.rdE) * vol;
+  }
+  // ---- y-edge row, inprod_y (src/aphid.f90:970-995)
+  if (j < ny - 1) {
+    const C k2y_c = a.kv(1, 0);
+    const R sm6 = Y.eP + Y.eM + X.nP + X.nM + Z.nP + Z.nM;
+    const C c1 = {k2y_c.re - sm6, k2y_c.im};
+    C acc = c1 * a.fv(1, 0);
+    acc += Y.eP * a.fv(1, 3);
+    acc += Y.eM * a.fv(1, 4);
+    acc += X.nP * a.fv(1, 1);
+    acc += X.nM * a.fv(1, 2);
+    acc += Z.nP * a.fv(1, 5);
+    acc += Z.nM * a.fv(1, 6);
+    const C cpl = a.fv(3, 3) * (a.kv(3, 3) - k2y_c) - a.fv(3, 0) * (k2p_c - k2y_c);
+    const R vol = X.dN * Y.dE * Z.dN;
+    ry = (acc + cpl * Y.rdE) * vol;
+  }
+  // ---- z-edge row, inprod_z (src/aphid.f90:1098-1123)
+  if (k < nz - 1) {
+    const C k2z_c = a.kv(2, 0);
+    const R sm6 = Z.eP + Z.eM + X.nP + X.nM + Y.nP + Y.nM;
+    const C c1 = {k2z_c.re - sm6, k2z_c.im};
+    C acc = c1 * a.fv(2, 0);
+    acc += Z.eP * a.fv(2, 5);
+    acc += Z.eM * a.fv(2, 6);
+    acc += X.nP * a.fv(2, 1);
+    acc += X.nM * a.fv(2, 2);
+    acc += Y.nP * a.fv(2, 3);
+    acc += Y.nM * a.fv(2, 4);
+    const C cpl = a.fv(3, 5) * (a.kv(3, 5) - k2z_c) - a.fv(3, 0) * (k2p_c - k2z_c);
+    const R vol = X.dN * Y.dN * Z.dE;
+    rz = (acc + cpl * Z.rdE) * vol;
+  }
+  // ---- interior node row, inprod_p (src/aphid.f90:1229-1278)
+  if (j > 0 && j < ny - 1 && k > 0 && k < nz - 1 && i > 0 && i < nx - 1) {
+    const C s1 = k2p_c, s2 = a.kv(0, 0), s3 = a.kv(0, 2);
+    const C s4 = a.kv(1, 0), s5 = a.kv(1, 4);
+    const C s6 = a.kv(2, 0), s7 = a.kv(2, 6);
+    const C c2 = X.nP * s2, c3 = X.nM * s3, c4 = Y.nP * s4;
+    const C c5 = Y.nM * s5, c6 = Z.nP * s6, c7 = Z.nM * s7;
+    const C c1 = s1 * s1 - (c2 + c3 + c4 + c5 + c6 + c7);
+    C acc = c1 * a.fv(3, 0);
+    acc += c2 * a.fv(3, 1);
+    acc += c3 * a.fv(3, 2);
+    acc += c4 * a.fv(3, 3);
+    acc += c5 * a.fv(3, 4);
+    acc += c6 * a.fv(3, 5);
+    acc += c7 * a.fv(3, 6);
+    C cp = ((s2 - s1) * X.rdN) * a.fv(0, 0);
+    cp += (-((s3 - s1) * X.rdN)) * a.fv(0, 2);
+    cp += ((s4 - s1) * Y.rdN) * a.fv(1, 0);
+    cp += (-((s5 - s1) * Y.rdN)) * a.fv(1, 4);
+    cp += ((s6 - s1) * Z.rdN) * a.fv(2, 0);
+    cp += (-((s7 - s1) * Z.rdN)) * a.fv(2, 6);
+    const R vol = X.dN * Y.dN * Z.dN;
+    rp = acc * vol + cp * vol;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void eps_acc(const cplx<R> (&in4)[4], cplx<R> rx, cplx<R> ry,
+                                        cplx<R> rz, cplx<R> rp, double& re, double& im) {
+  const cplx<double> a0 = ccast<double>(in4[0]) * ccast<double>(rx);
+  const cplx<double> a1 = ccast<double>(in4[1]) * ccast<double>(ry);
+  const cplx<double> a2 = ccast<double>(in4[2]) * ccast<double>(rz);
+  const cplx<double> a3 = ccast<double>(in4[3]) * ccast<double>(rp);
+  re += (a0.re + a1.re) + (a2.re + a3.re);
+  im += (a0.im + a1.im) + (a2.im + a3.im);
+}
+
+// The hardware hands consecutive block ids to the 8 XCDs round-robin; this
+// bijection gives XCD x the contiguous range of logical tiles
+// [x*q + min(x,r), ...), so that tiles next to each other share one L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+  const unsigned q = nb / 8, r = nb % 8, x = b % 8, loc = b / 8;
+  return x * q + min(x, r) + loc;
+}
+
+// ---------------------------------------------------------------------------
+// Variant 0: x-march.  A block owns TJ x TK columns and `chunk` x-planes;
+// x-neighbours in a per-thread register queue, in-plane neighbours from
+// the current plane staged in LDS.
+template <typename R, int TK, int TJ>
+struct MarchAcc {
+  using C = cplx<R>;
+  static constexpr int LK = TK + 2;
+  const C (*sm)[(TK + 2) * (TJ + 2)];
+  const C* pa;
+  const C* ca;
+  const C* na;
+  int ol;
+  C k2x_p, k2x_c, k2p_c, k2p_n;
+  __device__ __forceinline__ C fv(int c, int d) const {
+    switch (d) {
+      case 0: return ca[c];
+      case 1: return na[c];
+      case 2: return pa[c];
+      case 3: return sm[c][ol + LK];
+      case 4: return sm[c][ol - LK];
+      case 5: return sm[c][ol + 1];
+      default: return sm[c][ol - 1];
+    }
+  }
+  __device__ __forceinline__ C kv(int c, int d) const {
+    if (c == 0) return d == 2 ? k2x_p : k2x_c;
+    if (c == 3) {
+      if (d == 0) return k2p_c;
+      if (d == 1) return k2p_n;
+      return d == 3 ? sm[6][ol + LK] : sm[6][ol + 1];
+    }
+    if (c == 1) return d == 4 ? sm[4][ol - LK] : sm[4][ol];
+    return d == 6 ? sm[5][ol - 1] : sm[5][ol];
+  }
+};
+
 template <typename R, int TK, int TJ>
 __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* __restrict__ in,
                                                   cplx<R>* __restrict__ out, int chunk,
@@ -80,9 +234,16 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const unsigned pl = (unsigned)ny * nz;
   const unsigned nnb = (unsigned)op.nn * CS;  // component stride in bytes
+  // 1-D grid; tiles ordered k-tile fastest, then j-tile, then x-chunk, and
+  // handed to the XCDs in contiguous ranges (xcd_remap) so neighbouring
+  // tiles share one L2
+  const int ntk = (nz + TK - 1) / TK, ntj = (ny + TJ - 1) / TJ;
+  const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+  const int bk = (int)(t % ntk), bj = (int)((t / ntk) % ntj);
+  const int bi = (int)(t / ((unsigned)ntk * ntj));
   const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
-  const int k = blockIdx.x * TK + tk, j = blockIdx.y * TJ + tj;
-  const int i0 = blockIdx.z * chunk;
+  const int k = bk * TK + tk, j = bj * TJ + tj;
+  const int i0 = bi * chunk;
   const int i1 = min(nx, i0 + chunk);
   const bool own = (j < ny) && (k < nz);
   const unsigned colb = own ? ((unsigned)j * nz + k) * CS : 0u;
@@ -90,28 +251,22 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   const C zero = {R(0), R(0)};
 
   if (threadIdx.x < TJ) {
-    const int jj = blockIdx.y * TJ + threadIdx.x;
+    const int jj = bj * TJ + threadIdx.x;
     const bool v = jj < ny;
-    tyj[0][threadIdx.x] = v ? op.nP[1][jj] : R(0);
-    tyj[1][threadIdx.x] = v ? op.nM[1][jj] : R(0);
-    tyj[2][threadIdx.x] = v ? op.eP[1][jj] : R(0);
-    tyj[3][threadIdx.x] = v ? op.eM[1][jj] : R(0);
-    tyj[4][threadIdx.x] = v ? op.dN[1][jj] : R(0);
-    tyj[5][threadIdx.x] = v ? op.dE[1][jj] : R(0);
-    tyj[6][threadIdx.x] = v ? op.rdN[1][jj] : R(0);
-    tyj[7][threadIdx.x] = v ? op.rdE[1][jj] : R(0);
+    const Ax1<R> t = v ? axis_at(op, 1, jj) : Ax1<R>{};
+    tyj[0][threadIdx.x] = t.nP; tyj[1][threadIdx.x] = t.nM;
+    tyj[2][threadIdx.x] = t.eP; tyj[3][threadIdx.x] = t.eM;
+    tyj[4][threadIdx.x] = t.dN; tyj[5][threadIdx.x] = t.dE;
+    tyj[6][threadIdx.x] = t.rdN; tyj[7][threadIdx.x] = t.rdE;
   }
   if (threadIdx.x < TK) {
-    const int kk = blockIdx.x * TK + threadIdx.x;
+    const int kk = bk * TK + threadIdx.x;
     const bool v = kk < nz;
-    tzk[0][threadIdx.x] = v ? op.nP[2][kk] : R(0);
-    tzk[1][threadIdx.x] = v ? op.nM[2][kk] : R(0);
-    tzk[2][threadIdx.x] = v ? op.eP[2][kk] : R(0);
-    tzk[3][threadIdx.x] = v ? op.eM[2][kk] : R(0);
-    tzk[4][threadIdx.x] = v ? op.dN[2][kk] : R(0);
-    tzk[5][threadIdx.x] = v ? op.dE[2][kk] : R(0);
-    tzk[6][threadIdx.x] = v ? op.rdN[2][kk] : R(0);
-    tzk[7][threadIdx.x] = v ? op.rdE[2][kk] : R(0);
+    const Ax1<R> t = v ? axis_at(op, 2, kk) : Ax1<R>{};
+    tzk[0][threadIdx.x] = t.nP; tzk[1][threadIdx.x] = t.nM;
+    tzk[2][threadIdx.x] = t.eP; tzk[3][threadIdx.x] = t.eM;
+    tzk[4][threadIdx.x] = t.dN; tzk[5][threadIdx.x] = t.dE;
+    tzk[6][threadIdx.x] = t.rdN; tzk[7][threadIdx.x] = t.rdE;
   }
 
   // own value of component c (0..3) or k2 grid c (0..3) at plane i
@@ -130,7 +285,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     else if (h < 2 * TK + TJ) { hj = h - 2 * TK; hk = -1; }
     else { hj = h - 2 * TK - TJ; hk = TK; }
   }
-  const int gj = blockIdx.y * TJ + hj, gk = blockIdx.x * TK + hk;
+  const int gj = bj * TJ + hj, gk = bk * TK + hk;
   const bool hin = hv && gj >= 0 && gj < ny && gk >= 0 && gk < nz;
   const unsigned hcolb = hin ? ((unsigned)gj * nz + gk) * CS : 0u;
   const int hl = (hj + 1) * LK + (hk + 1);
@@ -170,96 +325,19 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
 
     if (own) {
       const unsigned ob = (unsigned)i * plb + colb;
-      const R nPx = op.nP[0][i], nMx = op.nM[0][i], dNx = op.dN[0][i];
-      const R nPy = tyj[0][tj], nMy = tyj[1][tj], nPz = tzk[0][tk], nMz = tzk[1][tk];
-      const R dNy = tyj[4][tj], dNz = tzk[4][tk];
-      const C k2y_c = sm[4][ol], k2z_c = sm[5][ol];
-      C rx = zero, ry = zero, rz = zero, rp = zero;
-      // ---- x-edge row, inprod_x (src/aphid.f90:841-866)
-      if (i < nx - 1) {
-        const R cE = op.eP[0][i], cW = op.eM[0][i];
-        const R sm6 = cE + cW + nPy + nMy + nPz + nMz;
-        const C c1 = {k2x_c.re - sm6, k2x_c.im};
-        C acc = c1 * ca[0];
-        acc += cE * na[0];
-        acc += cW * pa[0];
-        acc += nPy * sm[0][ol + LK];
-        acc += nMy * sm[0][ol - LK];
-        acc += nPz * sm[0][ol + 1];
-        acc += nMz * sm[0][ol - 1];
-        const C cpl = na[3] * (k2p_n - k2x_c) - ca[3] * (k2p_c - k2x_c);
-        const R vol = op.dE[0][i] * dNy * dNz;
-        rx = (acc + cpl * op.rdE[0][i]) * vol;
-      }
+      const Ax1<R> X = axis_at(op, 0, i);
+      const Ax1<R> Y = {tyj[0][tj], tyj[1][tj], tyj[2][tj], tyj[3][tj],
+                        tyj[4][tj], tyj[5][tj], tyj[6][tj], tyj[7][tj]};
+      const Ax1<R> Z = {tzk[0][tk], tzk[1][tk], tzk[2][tk], tzk[3][tk],
+                        tzk[4][tk], tzk[5][tk], tzk[6][tk], tzk[7][tk]};
+      MarchAcc<R, TK, TJ> a{sm, pa, ca, na, ol, k2x_p, k2x_c, k2p_c, k2p_n};
+      C rx, ry, rz, rp;
+      rows4<R>(i, j, k, nx, ny, nz, X, Y, Z, a, rx, ry, rz, rp);
       sto(out, ob, rx);
-      // ---- y-edge row, inprod_y (src/aphid.f90:970-995)
-      if (j < ny - 1) {
-        const R cN = tyj[2][tj], cS = tyj[3][tj];
-        const R sm6 = cN + cS + nPx + nMx + nPz + nMz;
-        const C c1 = {k2y_c.re - sm6, k2y_c.im};
-        C acc = c1 * ca[1];
-        acc += cN * sm[1][ol + LK];
-        acc += cS * sm[1][ol - LK];
-        acc += nPx * na[1];
-        acc += nMx * pa[1];
-        acc += nPz * sm[1][ol + 1];
-        acc += nMz * sm[1][ol - 1];
-        const C cpl = sm[3][ol + LK] * (sm[6][ol + LK] - k2y_c) - ca[3] * (k2p_c - k2y_c);
-        const R vol = dNx * tyj[5][tj] * dNz;
-        ry = (acc + cpl * tyj[7][tj]) * vol;
-      }
       sto(out, nnb + ob, ry);
-      // ---- z-edge row, inprod_z (src/aphid.f90:1098-1123)
-      if (k < nz - 1) {
-        const R cU = tzk[2][tk], cD = tzk[3][tk];
-        const R sm6 = cU + cD + nPx + nMx + nPy + nMy;
-        const C c1 = {k2z_c.re - sm6, k2z_c.im};
-        C acc = c1 * ca[2];
-        acc += cU * sm[2][ol + 1];
-        acc += cD * sm[2][ol - 1];
-        acc += nPx * na[2];
-        acc += nMx * pa[2];
-        acc += nPy * sm[2][ol + LK];
-        acc += nMy * sm[2][ol - LK];
-        const C cpl = sm[3][ol + 1] * (sm[6][ol + 1] - k2z_c) - ca[3] * (k2p_c - k2z_c);
-        const R vol = dNx * dNy * tzk[5][tk];
-        rz = (acc + cpl * tzk[7][tk]) * vol;
-      }
       sto(out, 2 * nnb + ob, rz);
-      // ---- interior node row, inprod_p (src/aphid.f90:1229-1278)
-      if (j > 0 && j < ny - 1 && k > 0 && k < nz - 1 && i > 0 && i < nx - 1) {
-        const C s1 = k2p_c, s2 = k2x_c, s3 = k2x_p;
-        const C s4 = k2y_c, s5 = sm[4][ol - LK];
-        const C s6 = k2z_c, s7 = sm[5][ol - 1];
-        const C c2 = nPx * s2, c3 = nMx * s3, c4 = nPy * s4;
-        const C c5 = nMy * s5, c6 = nPz * s6, c7 = nMz * s7;
-        const C c1 = s1 * s1 - (c2 + c3 + c4 + c5 + c6 + c7);
-        C acc = c1 * ca[3];
-        acc += c2 * na[3];
-        acc += c3 * pa[3];
-        acc += c4 * sm[3][ol + LK];
-        acc += c5 * sm[3][ol - LK];
-        acc += c6 * sm[3][ol + 1];
-        acc += c7 * sm[3][ol - 1];
-        const R rx_ = op.rdN[0][i], ry_ = tyj[6][tj], rz_ = tzk[6][tk];
-        C cp = ((s2 - s1) * rx_) * ca[0];
-        cp += (-((s3 - s1) * rx_)) * pa[0];
-        cp += ((s4 - s1) * ry_) * ca[1];
-        cp += (-((s5 - s1) * ry_)) * sm[1][ol - LK];
-        cp += ((s6 - s1) * rz_) * ca[2];
-        cp += (-((s7 - s1) * rz_)) * sm[2][ol - 1];
-        const R vol = dNx * dNy * dNz;
-        rp = acc * vol + cp * vol;
-      }
       sto(out, 3 * nnb + ob, rp);
-      if (ep.st) {  // eps partial: sum_c in_c * out_c in FP64
-        const cplx<double> a0 = ccast<double>(ca[0]) * ccast<double>(rx);
-        const cplx<double> a1 = ccast<double>(ca[1]) * ccast<double>(ry);
-        const cplx<double> a2 = ccast<double>(ca[2]) * ccast<double>(rz);
-        const cplx<double> a3 = ccast<double>(ca[3]) * ccast<double>(rp);
-        eacc_re += (a0.re + a1.re) + (a2.re + a3.re);
-        eacc_im += (a0.im + a1.im) + (a2.im + a3.im);
-      }
+      if (ep.st) eps_acc<R>(ca, rx, ry, rz, rp, eacc_re, eacc_im);
     }
     __syncthreads();
 #pragma unroll
@@ -272,6 +350,86 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     k2p_c = k2p_n;
   }
 
+  if (ep.st) {
+    double v[2] = {eacc_re, eacc_im};
+    block_sum<2>(v);
+    block_partial<2>(v, ep.part);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 1: direct.  One node per thread, every neighbour read through the
+// L1/L2 caches; blocks are TK x TJ tiles of one x-plane.  The linear block id
+// is remapped so that the blocks one XCD receives (ids congruent mod 8) walk
+// one contiguous slab of planes in order: neighbouring planes are then
+// re-read from that XCD's own L2.
+template <typename R>
+struct DirectAcc {
+  using C = cplx<R>;
+  const C* __restrict__ in;
+  const C* __restrict__ k2;
+  unsigned nnb, plb, rowb;  // component, plane and row (j) strides in bytes
+  unsigned ob;              // byte offset of the node
+  int i, j, k, nx, ny, nz;
+  __device__ __forceinline__ bool inside(int d) const {
+    switch (d) {
+      case 1: return i + 1 < nx;
+      case 2: return i > 0;
+      case 3: return j + 1 < ny;
+      case 4: return j > 0;
+      case 5: return k + 1 < nz;
+      case 6: return k > 0;
+      default: return true;
+    }
+  }
+  __device__ __forceinline__ unsigned off(int d) const {
+    switch (d) {
+      case 1: return ob + plb;
+      case 2: return ob - plb;
+      case 3: return ob + rowb;
+      case 4: return ob - rowb;
+      case 5: return ob + (unsigned)sizeof(C);
+      case 6: return ob - (unsigned)sizeof(C);
+      default: return ob;
+    }
+  }
+  __device__ __forceinline__ C fv(int c, int d) const {
+    return inside(d) ? ldo(in, c * nnb + off(d)) : C{R(0), R(0)};
+  }
+  __device__ __forceinline__ C kv(int c, int d) const {
+    return inside(d) ? ldo(k2, c * nnb + off(d)) : C{R(0), R(0)};
+  }
+};
+
+template <typename R, int TK, int TJ>
+__global__ __launch_bounds__(TK* TJ) void k_apply_direct(OpArgs<R> op, const cplx<R>* __restrict__ in,
+                                                         cplx<R>* __restrict__ out, ApplyEpilogue ep) {
+  using C = cplx<R>;
+  if (ep.st && ep.st->done) return;
+  const int nx = op.nx, ny = op.ny, nz = op.nz;
+  const int ntk = (nz + TK - 1) / TK, ntj = (ny + TJ - 1) / TJ;
+  const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+  const int tkt = (int)(t % ntk);
+  const int tjt = (int)((t / ntk) % ntj);
+  const int i = (int)(t / ((unsigned)ntk * ntj));
+  const int k = tkt * TK + (int)(threadIdx.x % TK), j = tjt * TJ + (int)(threadIdx.x / TK);
+  double eacc_re = 0.0, eacc_im = 0.0;
+  if (j < ny && k < nz && i < nx) {
+    constexpr unsigned CS = sizeof(C);
+    DirectAcc<R> a{in, op.k2, (unsigned)op.nn * CS, (unsigned)ny * nz * CS, (unsigned)nz * CS,
+                   (((unsigned)i * ny + j) * nz + k) * CS, i, j, k, nx, ny, nz};
+    C rx, ry, rz, rp;
+    rows4<R>(i, j, k, nx, ny, nz, axis_at(op, 0, i), axis_at(op, 1, j), axis_at(op, 2, k), a, rx,
+             ry, rz, rp);
+    sto(out, a.ob, rx);
+    sto(out, a.nnb + a.ob, ry);
+    sto(out, 2 * a.nnb + a.ob, rz);
+    sto(out, 3 * a.nnb + a.ob, rp);
+    if (ep.st) {
+      const C in4[4] = {a.fv(0, 0), a.fv(1, 0), a.fv(2, 0), a.fv(3, 0)};
+      eps_acc<R>(in4, rx, ry, rz, rp, eacc_re, eacc_im);
+    }
+  }
   if (ep.st) {
     double v[2] = {eacc_re, eacc_im};
     block_sum<2>(v);

@@ -259,6 +259,13 @@ class Solver:
         _check(lib().aphid_time_apply(self._ctx, precision, reps, C.byref(s)), "aphid_time_apply")
         return s.value
 
+    def time_copy(self, nbytes: float, reps: int = 10) -> float:
+        """Seconds per plain device copy of nbytes (reads nbytes, writes nbytes)."""
+        t = C.c_double()
+        _check(lib().aphid_time_copy(self._ctx, C.c_double(nbytes), reps, C.byref(t)),
+               "aphid_time_copy")
+        return t.value
+
     def time_iteration(self, precision: int = 0, reps: int = 10):
         """[p-update, stencil+eps, ||t-beta v||, fused update, iteration] seconds."""
         out = np.zeros(5)
@@ -271,4 +278,4 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
            "aphid_get_diag", "aphid_apply", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
            "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
-           "aphid_fetch_solution", "aphid_time_iteration"]
+           "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy"]

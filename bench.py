@@ -99,6 +99,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-its", type=int, default=30)
+    ap.add_argument("--max-it", type=int, default=MAX_IT,
+                    help="iteration cap per solve (profiling runs use a small cap; "
+                         "a capped solve does not count as converged)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); systems of this solve "
@@ -116,14 +119,14 @@ def main():
     b = s.setup_case(case)
     s.load_rhs(b)
     for w in range(args.warmup):
-        r = s.solve_resident(TOL, MAX_IT)
+        r = s.solve_resident(TOL, args.max_it)
         log(f"[bench] warmup {w}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
             f"{r.seconds_qmr:.3f}s")
 
     results = []
     t0 = time.perf_counter()  # each solve returns after its stream has drained
     for k in range(args.steps):
-        r = s.solve_resident(TOL, MAX_IT)
+        r = s.solve_resident(TOL, args.max_it)
         results.append(r)
         log(f"[bench] step {k}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
             f"{r.seconds_qmr:.3f}s")
@@ -146,6 +149,11 @@ def main():
     kernels = {k: {"algorithmic_bytes": int(bb), "seconds": t, "GBps": bb / t / 1e9,
                    "frac": bb / t / 1e9 / HBM_PEAK_GBS} for k, (bb, t) in per_kernel.items()}
     dom = max(per_kernel, key=lambda k: per_kernel[k][1])
+    copy_bytes = float(1 << 30)
+    t_copy = s.time_copy(copy_bytes, 10)
+    copy_probe = {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "seconds": t_copy,
+                  "GBps": 2 * copy_bytes / t_copy / 1e9,
+                  "frac_of_peak": 2 * copy_bytes / t_copy / 1e9 / HBM_PEAK_GBS}
     it_bytes = 22 * M * s_bytes
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_config2.json")
@@ -192,6 +200,7 @@ def main():
         "iteration": {"seconds": kt[4], "algorithmic_bytes": it_bytes,
                       "GBps": it_bytes / kt[4] / 1e9,
                       "frac": it_bytes / kt[4] / 1e9 / HBM_PEAK_GBS},
+        "hbm_copy_probe": copy_probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

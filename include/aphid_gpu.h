@@ -152,9 +152,11 @@ int aphid_solve_resident(aphid_ctx *ctx, double tol, int max_it, unsigned flags,
 int aphid_fetch_solution(aphid_ctx *ctx, double *x_x, double *x_y, double *x_z,
                          double *x_p);
 
-/* Stencil launch shape in use (set by the autotuner on aphid_set_mesh, or
- * fixed by aphid_set_tile): tile along z (tk), along y (tj), x planes per
- * block (chunk). precision: 0 = complex128 kernel, 1 = complex64 kernel. */
+/* Stencil launch shape in use (picked by the autotuner at the first apply or
+ * solve on a mesh, or fixed by aphid_set_tile): tile along z (tk), along y
+ * (tj), and x planes per block (chunk >= 1: the x-marching kernel with
+ * LDS-staged planes; chunk = 0: the one-node-per-thread kernel).
+ * precision: 0 = complex128 kernel, 1 = complex64 kernel. */
 int aphid_get_tile(aphid_ctx *ctx, int precision, int *tk, int *tj, int *chunk);
 int aphid_set_tile(aphid_ctx *ctx, int precision, int tk, int tj, int chunk);
 
@@ -168,6 +170,11 @@ int aphid_time_apply(aphid_ctx *ctx, int precision, int reps, double *seconds);
  * the eps reduction, [2] ||t - beta v||, [3] fused update, [4] the whole
  * iteration.  Overwrites the resident iterate. */
 int aphid_time_iteration(aphid_ctx *ctx, int precision, int reps, double *secs);
+
+/* Mean device time of a plain device-to-device copy kernel over `bytes`
+ * (read `bytes`, write `bytes`): the achievable-HBM-bandwidth probe that
+ * calibrates the roofline and the hardware traffic counters. */
+int aphid_time_copy(aphid_ctx *ctx, double bytes, int reps, double *seconds);
 
 #ifdef __cplusplus
 }

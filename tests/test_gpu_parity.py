@@ -85,20 +85,22 @@ def test_apply_matches_reference_goldens(gpu, path):
         assert relerr(gpu.get_diag()[c].ravel(order="F"), z[f"diag_{name}"]) < 1e-14
 
 
-def test_every_tile_shape_gives_identical_results(gpu):
+def test_every_tile_shape_and_variant_gives_the_same_results(gpu):
     case = CASES["blocks_23x31x19"]()
     _setup(gpu, case)
     u = _rand4(case.shape, 7)
     ref = None
     for tk, tj in [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4)]:
-        for chunk in (1, 3, 8, 64):
+        for chunk in (0, 1, 3, 8, 64):
             print(f"[tile] {tk}x{tj} chunk {chunk}", flush=True)
             gpu.set_tile(tk, tj, chunk)
             out = flat4(gpu.apply(u))
             if ref is None:
                 ref = out
             else:
-                np.testing.assert_array_equal(out, ref)
+                # same arithmetic (one shared row function); the compiler may
+                # contract multiply-adds differently per instantiation
+                assert relerr(out, ref) < 1e-14
 
 
 @pytest.mark.parametrize("path", golden_files("rhs_"), ids=os.path.basename)
