@@ -1250,3 +1250,532 @@ int aphid_time_copy(aphid_ctx* c, double bytes, int reps, double* seconds) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched solve (include/aphid_gpu.h, aphid_batch_*): many systems, one
+// launch per phase.  Each operator (mesh + cells + omega) is built by an
+// internal single-system context, so the batch reuses the setup kernels
+// unchanged; its k^2 stays there and 1/diag is gathered into one flat array.
+#include "batch.hpp"
+
+struct aphid_batch {
+  int dev = 0;
+  hipStream_t s = nullptr;
+  std::vector<aphid_ctx*> ops;
+  std::vector<long long> opoff;  // element offset of each op in the flat 1/diag
+  struct Sys {
+    int op;
+    long long off, n4;
+    bool x0;
+    c128* staged;  // b (and x0) on the device until the flat arrays exist
+  };
+  std::vector<Sys> sys;
+  long long n4_tot = 0, op4_tot = 0;
+  // device
+  c128* V[9] = {};  // x b v p t d s r spare (flat over systems)
+  c128* invd = nullptr;
+  OpArgs<double>* dops = nullptr;
+  // complex64 inner solve (APHID_F_MIXED): e bi v p t d s r, 1/diag, operators
+  c64* F[8] = {};
+  c64* invdf = nullptr;
+  OpArgs<float>* dops32 = nullptr;
+  double *scale = nullptr, *itol = nullptr;
+  int* nopen = nullptr;
+  int* budget = nullptr;  // per-system inner iteration budget of the sweep
+  hipGraphExec_t gexec32 = nullptr;
+  int giters32 = 0;
+  long long *doff = nullptr, *dn4 = nullptr, *dopoff = nullptr;
+  int *dop = nullptr, *dbstart = nullptr, *dtstart = nullptr;
+  QmrState* st = nullptr;
+  std::vector<QmrState> hst;
+  double* part = nullptr;
+  double* res = nullptr;
+  int nstream = 0, ntile = 0;
+  bool built = false;
+  bool solved = false;
+  hipGraphExec_t gexec = nullptr;
+  int giters = 0;
+  c128* stage = nullptr;
+  long long stage_n = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+constexpr int kBT = 256;  // batched stencil block (flat in-plane chunk)
+
+void batch_free_device(aphid_batch* B) {
+  if (B->gexec) hipGraphExecDestroy(B->gexec), B->gexec = nullptr;
+  if (B->gexec32) hipGraphExecDestroy(B->gexec32), B->gexec32 = nullptr;
+  for (auto& v : B->F)
+    if (v) hipFree(v), v = nullptr;
+  void* ps32[] = {B->invdf, B->dops32, B->scale, B->itol, B->nopen, B->budget};
+  for (void* p : ps32)
+    if (p) hipFree(p);
+  B->invdf = nullptr; B->dops32 = nullptr; B->scale = nullptr; B->itol = nullptr; B->nopen = nullptr;
+  B->budget = nullptr;
+  for (auto& v : B->V)
+    if (v) hipFree(v), v = nullptr;
+  void* ps[] = {B->invd, B->dops, B->doff, B->dn4, B->dopoff, B->dop, B->dbstart, B->dtstart,
+                B->st, B->part, B->res};
+  for (void* p : ps)
+    if (p) hipFree(p);
+  B->invd = nullptr; B->dops = nullptr; B->doff = nullptr; B->dn4 = nullptr; B->dopoff = nullptr;
+  B->dop = nullptr; B->dbstart = nullptr; B->dtstart = nullptr; B->st = nullptr; B->part = nullptr;
+  B->res = nullptr;
+  B->built = false;
+}
+
+BatchDev batch_dev(const aphid_batch* B) {
+  return BatchDev{(int)B->sys.size(), B->doff, B->dn4, B->dop, B->dopoff, B->dbstart, B->dtstart,
+                  B->st};
+}
+
+// (re)build the flat device arrays after systems were added
+int batch_build(aphid_batch* B) {
+  if (B->built) return 0;
+  const int ns = (int)B->sys.size(), no = (int)B->ops.size();
+  if (B->V[1]) return fail(APHID_E_STATE, "internal: rebuild with live vectors");
+  std::vector<long long> off(ns), n4(ns), opoff(no);
+  std::vector<int> op(ns), bstart(ns + 1), tstart(ns + 1);
+  long long o = 0;
+  for (int q = 0; q < no; ++q) {
+    opoff[q] = o;
+    o += 4 * B->ops[q]->nn;
+  }
+  B->op4_tot = o;
+  o = 0;
+  int nb = 0, nt = 0;
+  for (int q = 0; q < ns; ++q) {
+    const aphid_ctx* c = B->ops[B->sys[q].op];
+    off[q] = o;
+    n4[q] = 4 * c->nn;
+    op[q] = B->sys[q].op;
+    B->sys[q].off = o;
+    B->sys[q].n4 = n4[q];
+    o += n4[q];
+    bstart[q] = nb;
+    nb += (int)((n4[q] + kBElems - 1) / kBElems);
+    tstart[q] = nt;
+    nt += c->nx * (int)((c->ny * c->nz + kBT - 1) / kBT);
+  }
+  bstart[ns] = nb;
+  tstart[ns] = nt;
+  B->n4_tot = o;
+  B->nstream = nb;
+  B->ntile = nt;
+  for (auto& v : B->V) CK(hipMalloc(&v, sizeof(c128) * B->n4_tot));
+  CK(hipMalloc(&B->invd, sizeof(c128) * B->op4_tot));
+  CK(hipMalloc(&B->dops, sizeof(OpArgs<double>) * no));
+  CK(hipMalloc(&B->doff, sizeof(long long) * ns));
+  CK(hipMalloc(&B->dn4, sizeof(long long) * ns));
+  CK(hipMalloc(&B->dopoff, sizeof(long long) * no));
+  CK(hipMalloc(&B->dop, sizeof(int) * ns));
+  CK(hipMalloc(&B->dbstart, sizeof(int) * (ns + 1)));
+  CK(hipMalloc(&B->dtstart, sizeof(int) * (ns + 1)));
+  CK(hipMalloc(&B->st, sizeof(QmrState) * ns));
+  CK(hipMalloc(&B->part, sizeof(double) * 3 * (size_t)std::max(nb, nt)));
+  CK(hipMalloc(&B->res, sizeof(double) * 2 * ns));
+  std::vector<OpArgs<double>> hops(no);
+  for (int q = 0; q < no; ++q) {
+    hops[q] = B->ops[q]->op64;
+    CK(hipMemcpy(B->invd + opoff[q], B->ops[q]->invd, sizeof(c128) * 4 * B->ops[q]->nn,
+                 hipMemcpyDeviceToDevice));
+  }
+  CK(hipMemcpy(B->dops, hops.data(), sizeof(OpArgs<double>) * no, hipMemcpyHostToDevice));
+  CK(hipMemcpy(B->doff, off.data(), sizeof(long long) * ns, hipMemcpyHostToDevice));
+  CK(hipMemcpy(B->dn4, n4.data(), sizeof(long long) * ns, hipMemcpyHostToDevice));
+  CK(hipMemcpy(B->dopoff, opoff.data(), sizeof(long long) * no, hipMemcpyHostToDevice));
+  CK(hipMemcpy(B->dop, op.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+  CK(hipMemcpy(B->dbstart, bstart.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+  CK(hipMemcpy(B->dtstart, tstart.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+  CK(hipMemset(B->V[0], 0, sizeof(c128) * B->n4_tot));
+  B->opoff = opoff;
+  B->hst.assign(ns, QmrState{});
+  B->built = true;
+  return 0;
+}
+
+template <typename V>
+int batch_enqueue_iteration(aphid_batch* B) {
+  const BatchDev D = batch_dev(B);
+  const int ns = (int)B->sys.size();
+  cplx<V>*x, *v, *p, *t, *d, *s, *r;
+  const cplx<V>* invd;
+  if constexpr (std::is_same<V, double>::value) {
+    x = B->V[0]; v = B->V[2]; p = B->V[3]; t = B->V[4]; d = B->V[5]; s = B->V[6]; r = B->V[7];
+    invd = B->invd;
+  } else {
+    x = B->F[0]; v = B->F[2]; p = B->F[3]; t = B->F[4]; d = B->F[5]; s = B->F[6]; r = B->F[7];
+    invd = B->invdf;
+  }
+  const OpArgs<V>* ops;
+  if constexpr (std::is_same<V, double>::value) ops = B->dops;
+  else ops = B->dops32;
+  hipLaunchKernelGGL(kb_pupdate<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, v, invd, p);
+  hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, p, t, 1,
+                     B->part);
+  hipLaunchKernelGGL(kb_fin_eps, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
+  hipLaunchKernelGGL(kb_vtnorm<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, t, v, B->part);
+  hipLaunchKernelGGL(kb_fin_vtnorm, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
+  hipLaunchKernelGGL(kb_update<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, p, d, x, t, s, r, v,
+                     invd, B->part);
+  hipLaunchKernelGGL(kb_fin_update, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
+  CK(hipGetLastError());
+  return 0;
+}
+
+bool batch_all_done(const aphid_batch* B) {
+  for (const QmrState& q : B->hst)
+    if (!q.done) return false;
+  return true;
+}
+
+int batch_fetch_states(aphid_batch* B) {
+  CK(hipMemcpyAsync(B->hst.data(), B->st, sizeof(QmrState) * B->hst.size(),
+                    hipMemcpyDeviceToHost, B->s));
+  CK(hipStreamSynchronize(B->s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aphid_batch_create(aphid_batch** out, int device) {
+  if (!out) return fail(APHID_E_ARG, "batch pointer is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  CK(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) return fail(APHID_E_HIP, "no HIP device");
+  if (device < 0) CK(hipGetDevice(&device));
+  if (device >= ndev) return fail(APHID_E_ARG, "device index out of range");
+  CK(hipSetDevice(device));
+  aphid_batch* B = new aphid_batch();
+  B->dev = device;
+  CK(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking));
+  CK(hipEventCreate(&B->ev0));
+  CK(hipEventCreate(&B->ev1));
+  *out = B;
+  return 0;
+}
+
+int aphid_batch_destroy(aphid_batch* B) {
+  if (!B) return 0;
+  hipSetDevice(B->dev);
+  if (B->s) hipStreamSynchronize(B->s);
+  batch_free_device(B);
+  if (B->stage) hipFree(B->stage);
+  for (auto& q : B->sys)
+    if (q.staged) hipFree(q.staged);
+  for (aphid_ctx* c : B->ops) aphid_ctx_destroy(c);
+  if (B->ev0) hipEventDestroy(B->ev0);
+  if (B->ev1) hipEventDestroy(B->ev1);
+  if (B->s) hipStreamDestroy(B->s);
+  delete B;
+  return 0;
+}
+
+int aphid_batch_add_operator(aphid_batch* B, int nx, int ny, int nz, const double* xn,
+                             const double* yn, const double* zn, char model, double omega,
+                             const float* a, const float* b, const float* c, const float* d) {
+  if (!B) return fail(APHID_E_ARG, "NULL batch");
+  if (!B->sys.empty()) return fail(APHID_E_STATE, "add every operator before the first system");
+  aphid_ctx* ctx = nullptr;
+  int rc = aphid_ctx_create(&ctx, B->dev);
+  if (rc) return rc;
+  rc = aphid_set_mesh(ctx, nx, ny, nz, xn, yn, zn);
+  if (!rc) rc = aphid_set_cells(ctx, model, omega, a, b, c, d);
+  if (rc) {
+    const std::string e = g_err;
+    aphid_ctx_destroy(ctx);
+    return fail(rc, e);
+  }
+  B->ops.push_back(ctx);
+  return (int)B->ops.size() - 1;
+}
+
+int aphid_batch_add_system(aphid_batch* B, int op, const double* bx, const double* by,
+                           const double* bz, const double* bp, const double* x0x,
+                           const double* x0y, const double* x0z, const double* x0p) {
+  if (!B || !bx || !by || !bz || !bp) return fail(APHID_E_ARG, "NULL argument");
+  if (op < 0 || op >= (int)B->ops.size()) return fail(APHID_E_ARG, "no such operator");
+  const bool have_x0 = x0x || x0y || x0z || x0p;
+  if (have_x0 && !(x0x && x0y && x0z && x0p)) return fail(APHID_E_ARG, "x0 arrays incomplete");
+  if (B->built) return fail(APHID_E_STATE, "add every system before the first solve");
+  B->sys.push_back({op, 0, 0, have_x0, nullptr});
+  // upload right-hand side and guess into host-side staging until the build
+  aphid_ctx* c = B->ops[op];
+  const long long n4 = 4 * c->nn;
+  c128* tmp = nullptr;
+  CK(hipSetDevice(B->dev));
+  CK(hipMalloc(&tmp, sizeof(c128) * n4 * (have_x0 ? 2 : 1)));
+  const double* hb[4] = {bx, by, bz, bp};
+  int rc = upload4(c, hb, tmp);
+  if (!rc && have_x0) {
+    const double* h0[4] = {x0x, x0y, x0z, x0p};
+    rc = upload4(c, h0, tmp + n4);
+  }
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c->s);
+    if (e != hipSuccess) rc = fail(APHID_E_HIP, hipGetErrorString(e));
+  }
+  if (rc) {
+    hipFree(tmp);
+    B->sys.pop_back();
+    return rc;
+  }
+  B->sys.back().staged = tmp;
+  return (int)B->sys.size() - 1;
+}
+
+}  // extern "C"
+
+namespace {
+
+// complex64 copies of the operators' data for the mixed-precision inner solve
+int batch_build_f32(aphid_batch* B) {
+  if (B->F[0]) return 0;
+  const int ns = (int)B->sys.size(), no = (int)B->ops.size();
+  for (auto& v : B->F) CK(hipMalloc(&v, sizeof(c64) * B->n4_tot));
+  CK(hipMalloc(&B->invdf, sizeof(c64) * B->op4_tot));
+  CK(hipMalloc(&B->dops32, sizeof(OpArgs<float>) * no));
+  CK(hipMalloc(&B->scale, sizeof(double) * ns));
+  CK(hipMalloc(&B->itol, sizeof(double) * ns));
+  CK(hipMalloc(&B->nopen, sizeof(int)));
+  CK(hipMalloc(&B->budget, sizeof(int) * ns));
+  std::vector<OpArgs<float>> hops(no);
+  for (int q = 0; q < no; ++q) {
+    hops[q] = B->ops[q]->op32;
+    CK(hipMemcpy(B->invdf + B->opoff[q], B->ops[q]->invdf, sizeof(c64) * 4 * B->ops[q]->nn,
+                 hipMemcpyDeviceToDevice));
+  }
+  CK(hipMemcpy(B->dops32, hops.data(), sizeof(OpArgs<float>) * no, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// One batched QMR solve at storage precision V: b, x and the Krylov vectors
+// of that precision; per-system stop targets itol[s]*||b_s|| (or tol).
+template <typename V>
+int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool use_x0,
+              const int* maxits = nullptr) {
+  constexpr bool f64 = std::is_same<V, double>::value;
+  const BatchDev D = batch_dev(B);
+  const int ns = (int)B->sys.size();
+  cplx<V>*x, *b, *v, *p, *t, *d, *s_, *r;
+  const cplx<V>* invd;
+  const OpArgs<V>* ops;
+  if constexpr (f64) {
+    x = B->V[0]; b = B->V[1]; v = B->V[2]; p = B->V[3]; t = B->V[4]; d = B->V[5]; s_ = B->V[6];
+    r = B->V[7]; invd = B->invd; ops = B->dops;
+  } else {
+    x = B->F[0]; b = B->F[1]; v = B->F[2]; p = B->F[3]; t = B->F[4]; d = B->F[5]; s_ = B->F[6];
+    r = B->F[7]; invd = B->invdf; ops = B->dops32;
+  }
+  CK(hipMemsetAsync(B->st, 0, sizeof(QmrState) * ns, B->s));
+  hipLaunchKernelGGL(kb_norm_b<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, b, B->part);
+  if (itol)
+    hipLaunchKernelGGL(kb_fin_norm_b_tol, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, itol);
+  else
+    hipLaunchKernelGGL(kb_fin_norm_b, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, tol);
+  if (use_x0)
+    hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, x, t, 0,
+                       B->part);
+  hipLaunchKernelGGL(kb_init_r<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, b,
+                     use_x0 ? (const cplx<V>*)t : nullptr, r, B->part);
+  hipLaunchKernelGGL(kb_fin_init_r, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
+  hipLaunchKernelGGL(kb_init_v<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, r, invd, v, p, d,
+                     s_, B->part);
+  hipLaunchKernelGGL(kb_fin_init_v, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, max_it, 0,
+                     maxits);
+  CK(hipGetLastError());
+  int rc = batch_fetch_states(B);
+  if (rc) return rc;
+  if (batch_all_done(B)) return 0;
+  const double bytes = 22.0 * (f64 ? 16.0 : 8.0) * (double)B->n4_tot;
+  const int k = std::max(4, std::min(128, (int)std::ceil(3e-3 / std::max(bytes / 4.0e12, 25e-6))));
+  hipGraphExec_t& gx = f64 ? B->gexec : B->gexec32;
+  int& gk = f64 ? B->giters : B->giters32;
+  if (!gx || gk != k) {
+    if (gx) hipGraphExecDestroy(gx), gx = nullptr;
+    if (use_graphs()) {
+      hipGraph_t g;
+      CK(hipStreamBeginCapture(B->s, hipStreamCaptureModeThreadLocal));
+      for (int q = 0; q < k && !rc; ++q) rc = batch_enqueue_iteration<V>(B);
+      hipError_t e = hipStreamEndCapture(B->s, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(APHID_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+      CK(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
+      CK(hipGraphDestroy(g));
+    }
+    gk = k;
+  }
+  const long long max_launch = (long long)max_it / k + 3;
+  for (long long l = 0;; ++l) {
+    if (gx) {
+      CK(hipGraphLaunch(gx, B->s));
+    } else {
+      for (int q = 0; q < k && !rc; ++q) rc = batch_enqueue_iteration<V>(B);
+      if (rc) return rc;
+    }
+    rc = batch_fetch_states(B);
+    if (rc) return rc;
+    if (batch_all_done(B)) return 0;
+    if (l > max_launch) return fail(APHID_E_STATE, "batched QMR state did not advance");
+  }
+}
+
+// per-system ||b - K x|| and ||b|| (complex128) into B->res
+int batch_true_residual(aphid_batch* B) {
+  const BatchDev D = batch_dev(B);
+  const int ns = (int)B->sys.size();
+  hipLaunchKernelGGL((kb_apply<double, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, B->dops,
+                     B->V[0], B->V[8], 0, B->part);
+  hipLaunchKernelGGL(kb_resnorm<double>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, B->V[1],
+                     B->V[8], B->part);
+  hipLaunchKernelGGL(kb_fin_resnorm, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, B->res);
+  CK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
+                      aphid_sys_report* reports, double* seconds) {
+  if (!B || B->sys.empty()) return fail(APHID_E_ARG, "empty batch");
+  if (!(tol >= 0.0) || max_it < 0) return fail(APHID_E_ARG, "tol and max_it must be >= 0");
+  CK(hipSetDevice(B->dev));
+  const int ns = (int)B->sys.size();
+  bool any_x0 = false;
+  if (!B->built) {
+    std::vector<c128*> staged(ns);
+    for (int q = 0; q < ns; ++q) staged[q] = B->sys[q].staged;
+    int rc = batch_build(B);
+    if (rc) return rc;
+    for (int q = 0; q < ns; ++q) {
+      const long long n4 = B->sys[q].n4;
+      CK(hipMemcpy(B->V[1] + B->sys[q].off, staged[q], sizeof(c128) * n4, hipMemcpyDeviceToDevice));
+      if (B->sys[q].x0)
+        CK(hipMemcpy(B->V[0] + B->sys[q].off, staged[q] + n4, sizeof(c128) * n4,
+                     hipMemcpyDeviceToDevice));
+      CK(hipFree(staged[q]));
+      B->sys[q].staged = nullptr;
+    }
+  }
+  // a guess given with the system applies to the first solve; later solves
+  // start from the current solutions only when asked (APHID_F_X0)
+  if (!B->solved)
+    for (const auto& q : B->sys) any_x0 |= q.x0;
+  const bool use_x0 = (flags & APHID_F_X0) || any_x0;
+  if (!use_x0) CK(hipMemsetAsync(B->V[0], 0, sizeof(c128) * B->n4_tot, B->s));
+  B->solved = true;
+  CK(hipEventRecord(B->ev0, B->s));
+  std::vector<int> iters(ns, 0), sweeps(ns, 0);
+  std::vector<QmrState> last(ns);
+  int rc = 0;
+  if (!(flags & APHID_F_MIXED)) {
+    rc = batch_qmr<double>(B, tol, nullptr, max_it, use_x0, nullptr);
+    if (rc) return rc;
+    last = B->hst;
+    for (int q = 0; q < ns; ++q) iters[q] = last[q].iter;
+  } else {
+    rc = batch_build_f32(B);
+    if (rc) return rc;
+    const BatchDev D = batch_dev(B);
+    const double floor_tol = mixed_inner_floor();
+    // each sweep may spend at most this many inner iterations per system (a
+    // complex64 Krylov process that stagnates is restarted from the refined
+    // FP64 iterate); every system also has its own total budget max_it
+    const int sweep_cap = std::max(200, std::max(max_it, 1) / 8);
+    std::vector<double> hscale(ns);
+    std::vector<int> budget(ns);
+    for (int sweep = 0; sweep < 64; ++sweep) {
+      rc = batch_true_residual(B);
+      if (rc) return rc;
+      int most = 0;
+      for (int q = 0; q < ns; ++q) {
+        budget[q] = std::min(sweep_cap, std::max(max_it, 1) - iters[q]);
+        most = std::max(most, budget[q]);
+      }
+      CK(hipMemcpyAsync(B->budget, budget.data(), sizeof(int) * ns, hipMemcpyHostToDevice, B->s));
+      CK(hipMemsetAsync(B->nopen, 0, sizeof(int), B->s));
+      hipLaunchKernelGGL(kb_inner_setup, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, B->s, ns,
+                         B->res, tol, floor_tol, B->budget, B->scale, B->itol, B->nopen);
+      int nopen = 0;
+      CK(hipMemcpyAsync(&nopen, B->nopen, sizeof(int), hipMemcpyDeviceToHost, B->s));
+      CK(hipMemcpyAsync(hscale.data(), B->scale, sizeof(double) * ns, hipMemcpyDeviceToHost, B->s));
+      CK(hipStreamSynchronize(B->s));
+      if (nopen == 0 || most <= 0) break;
+      hipLaunchKernelGGL(kb_inner_rhs, dim3(B->nstream), dim3(kBlock), 0, B->s, D, B->V[1],
+                         B->V[8], B->scale, B->F[1], B->F[0]);
+      CK(hipGetLastError());
+      rc = batch_qmr<float>(B, 0.0, B->itol, most, false, B->budget);
+      if (rc) return rc;
+      for (int q = 0; q < ns; ++q) {
+        if (hscale[q] == 0.0) continue;
+        const QmrState& h = B->hst[q];
+        iters[q] += h.status == ST_MAXIT ? std::max(budget[q], 1) : std::max(h.iter, 0);
+        ++sweeps[q];
+        last[q] = h;
+      }
+      hipLaunchKernelGGL(kb_refine, dim3(B->nstream), dim3(kBlock), 0, B->s, D, B->V[0], B->F[0],
+                         B->res, B->scale);
+      CK(hipGetLastError());
+    }
+  }
+  CK(hipEventRecord(B->ev1, B->s));
+  CK(hipEventSynchronize(B->ev1));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+  rc = batch_true_residual(B);
+  if (rc) return rc;
+  std::vector<double> res(2 * ns);
+  CK(hipMemcpyAsync(res.data(), B->res, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, B->s));
+  CK(hipStreamSynchronize(B->s));
+  int worst = 0;
+  for (int q = 0; q < ns; ++q) {
+    const double rel = res[2 * q + 1] > 0 ? res[2 * q] / res[2 * q + 1] : res[2 * q];
+    int status = last[q].status;
+    if (flags & APHID_F_MIXED)
+      status = (rel < tol) ? ST_CONV : (std::isfinite(rel) ? ST_MAXIT : ST_BREAK);
+    if (reports) {
+      reports[q].iters = iters[q];
+      reports[q].status = status;
+      reports[q].rnorm = (flags & APHID_F_MIXED) ? res[2 * q] : last[q].rnorm;
+      reports[q].rhsnorm = res[2 * q + 1];
+      reports[q].true_rel_res = rel;
+    }
+    worst = std::max(worst, status);
+  }
+  if (seconds) *seconds = 1e-3 * ms;
+  return worst;
+}
+
+int aphid_batch_get_solution(aphid_batch* B, int sys, double* x_x, double* x_y, double* x_z,
+                             double* x_p) {
+  if (!B || !x_x || !x_y || !x_z || !x_p) return fail(APHID_E_ARG, "NULL argument");
+  if (sys < 0 || sys >= (int)B->sys.size() || !B->built) return fail(APHID_E_ARG, "no such solved system");
+  CK(hipSetDevice(B->dev));
+  aphid_ctx* c = B->ops[B->sys[sys].op];
+  double* h[4] = {x_x, x_y, x_z, x_p};
+  return download4(c, B->V[0] + B->sys[sys].off, h);
+}
+
+int aphid_batch_set_guess(aphid_batch* B, int sys, int from_sys) {
+  if (!B || !B->built) return fail(APHID_E_STATE, "solve the batch once first");
+  const int ns = (int)B->sys.size();
+  if (sys < 0 || sys >= ns || from_sys < 0 || from_sys >= ns) return fail(APHID_E_ARG, "no such system");
+  if (B->sys[sys].op != B->sys[from_sys].op &&
+      B->ops[B->sys[sys].op]->nn != B->ops[B->sys[from_sys].op]->nn)
+    return fail(APHID_E_ARG, "guess must come from a system on the same mesh");
+  if (sys == from_sys) return 0;
+  CK(hipSetDevice(B->dev));
+  CK(hipMemcpyAsync(B->V[0] + B->sys[sys].off, B->V[0] + B->sys[from_sys].off,
+                    sizeof(c128) * B->sys[sys].n4, hipMemcpyDeviceToDevice, B->s));
+  CK(hipStreamSynchronize(B->s));
+  return 0;
+}
+
+}  // extern "C"

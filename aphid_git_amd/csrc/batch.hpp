@@ -1,0 +1,365 @@
+// batch.hpp -- many independent A-Phi systems in one launch per phase.
+//
+// The reference solves one (frequency, source) system per process and farms
+// them out with MPI (src/aphid_parallel.f90:75-201).  Here all systems of a
+// batch -- different frequencies, sources and mesh sizes -- share every
+// launch of the Jacobi-QMR iteration:
+//   - all vectors of all systems are concatenated in one flat array per
+//     vector kind; system s owns [off_s, off_s + n4_s) (n4_s = 4 nn_s,
+//     device layout of stencil.hpp);
+//   - streaming kernels run a work list of fixed-size element ranges, each
+//     range inside one system (block b -> system via binary search of the
+//     per-system block prefix); per-block partials are summed by one
+//     finalize block per system;
+//   - the stencil runs a work list of (system, tile) items over all systems'
+//     planes (one node per thread, neighbours through L1/L2);
+//   - systems with the same mesh and frequency share one operator (k^2 and
+//     1/diag); every system carries its own QmrState, and blocks of a
+//     converged system return at once (the convergence mask).
+#pragma once
+#include "qmr.hpp"
+#include "stencil.hpp"
+
+constexpr int kBElems = 4096;  // elements per streaming work item (256 threads x 16)
+
+struct BatchDev {
+  int nsys;
+  const long long* off;     // [nsys] element offset of each system's block
+  const long long* n4;      // [nsys] 4*nn of each system
+  const int* op;            // [nsys] operator index
+  const long long* opoff;   // [nops] element offset of each operator's k2 / 1/diag block
+  const int* bstart;        // [nsys+1] streaming work-item prefix
+  const int* tstart;        // [nsys+1] stencil tile prefix
+  QmrState* st;             // [nsys]
+};
+
+__device__ __forceinline__ int find_sys(const int* __restrict__ start, int nsys, int b) {
+  int lo = 0, hi = nsys;  // start[lo] <= b < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// streaming work item of this block: system, global element range [e0, e1)
+struct BItem {
+  int s;
+  long long e0, e1;
+};
+__device__ __forceinline__ BItem b_item(const BatchDev& B) {
+  const int b = blockIdx.x;
+  const int s = find_sys(B.bstart, B.nsys, b);
+  const long long lb = b - B.bstart[s];
+  const long long e0 = lb * kBElems, e1 = min(e0 + kBElems, B.n4[s]);
+  return {s, B.off[s] + e0, B.off[s] + e1};
+}
+
+#define B_LOOP(q, it) for (long long q = (it).e0 + threadIdx.x; q < (it).e1; q += blockDim.x)
+
+template <typename V>
+__global__ void kb_norm_b(BatchDev B, const vec<V>* __restrict__ b, double* part) {
+  const BItem it = b_item(B);
+  double a[1] = {0.0};
+  B_LOOP(q, it) a[0] += abs2(up(b[q]));
+  block_sum<1>(a);
+  block_partial<1>(a, part);
+}
+
+// finalize helper: fixed-order sum of system s's work-item partials
+template <int NV>
+__device__ __forceinline__ void b_fin_sum(const double* __restrict__ part, int p0, int p1,
+                                          double (&tot)[NV]) {
+  finalize_sum<NV>(part + (size_t)p0 * NV, p1 - p0, tot);
+}
+
+__global__ void kb_fin_norm_b(BatchDev B, const double* part, double tol) {
+  const int s = blockIdx.x;
+  double tot[1];
+  b_fin_sum<1>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) {
+    B.st[s].rhsnorm = sqrt(tot[0]);
+    B.st[s].target = tol * B.st[s].rhsnorm;
+  }
+}
+
+// r = b - t (t = K x0 when warm-started), ||r||^2 partials
+template <typename V>
+__global__ void kb_init_r(BatchDev B, const vec<V>* __restrict__ b, const vec<V>* __restrict__ t,
+                          vec<V>* __restrict__ r, double* part) {
+  const BItem it = b_item(B);
+  double a[1] = {0.0};
+  B_LOOP(q, it) {
+    c128 e = up(b[q]);
+    if (t) e = e - up(t[q]);
+    const vec<V> ev = dn<V>(e);
+    r[q] = ev;
+    a[0] += abs2(up(ev));
+  }
+  block_sum<1>(a);
+  block_partial<1>(a, part);
+}
+__global__ void kb_fin_init_r(BatchDev B, const double* part) {
+  const int s = blockIdx.x;
+  double tot[1];
+  b_fin_sum<1>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) B.st[s].rho_np = sqrt(tot[0]);
+}
+
+template <typename V>
+__global__ void kb_init_v(BatchDev B, const vec<V>* __restrict__ r, const vec<V>* __restrict__ invd,
+                          vec<V>* __restrict__ v, vec<V>* __restrict__ p, vec<V>* __restrict__ d,
+                          vec<V>* __restrict__ s_, double* part) {
+  const BItem it = b_item(B);
+  const double rho = B.st[it.s].rho_np;
+  const long long io = B.opoff[B.op[it.s]] - B.off[it.s];  // system -> operator offset
+  const vec<V> z = {V(0), V(0)};
+  double a[2] = {0.0, 0.0};
+  B_LOOP(q, it) {
+    const vec<V> vv = dn<V>(divr(up(r[q]), rho));
+    v[q] = vv;
+    p[q] = z;
+    d[q] = z;
+    s_[q] = z;
+    const c128 vd = up(vv);
+    const c128 pr = vd * (vd * up(invd[q + io]));
+    a[0] += pr.re;
+    a[1] += pr.im;
+  }
+  block_sum<2>(a);
+  block_partial<2>(a, part);
+}
+__global__ void kb_fin_init_v(BatchDev B, const double* part, int max_it, int hist_cap,
+                              const int* __restrict__ maxits) {
+  const int s = blockIdx.x;
+  double tot[2];
+  b_fin_sum<2>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) {
+    QmrState* st = &B.st[s];
+    const double rho = st->rho_np;
+    st->max_it = maxits ? maxits[s] : max_it;
+    st->hist_cap = hist_cap;
+    st->c_n = 1.0;
+    st->eps = {1.0, 0.0};
+    st->theta_n = 0.0;
+    st->eta = {-1.0, 0.0};
+    st->iter = 1;
+    st->done = 0;
+    st->status = ST_MAXIT;
+    st->rnorm = rho;
+    st->c_nm = st->c_n;
+    st->rho_n = rho;
+    st->theta_nm = st->theta_n;
+    st->del = {tot[0], tot[1]};
+    st->k1 = cdiv(rho * st->del, st->eps);
+    if (!(rho > 0.0) || !isfinite(rho)) {
+      st->done = 1;
+      st->status = (rho == 0.0) ? ST_CONV : ST_BREAK;
+      st->iter = 0;
+    } else if (is_zero(st->del) || !finite_c(st->k1)) {
+      st->done = 1;
+      st->status = ST_BREAK;
+    }
+  }
+}
+
+template <typename V>
+__global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
+                           vec<V>* __restrict__ p) {
+  const BItem it = b_item(B);
+  const QmrState* st = &B.st[it.s];
+  if (st->done) return;
+  const c128 k1 = st->k1;
+  const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
+  B_LOOP(q, it) p[q] = dn<V>(up(v[q]) * up(invd[q + io]) - up(p[q]) * k1);
+}
+
+// batched stencil: one node per thread, work list of (system, x-plane,
+// flat in-plane chunk of NT nodes) over all systems (flat in-plane chunks fit
+// any (ny, nz) without idle lanes); eps partial per work item
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __restrict__ ops,
+                                               const cplx<R>* __restrict__ in,
+                                               cplx<R>* __restrict__ out, int with_eps,
+                                               double* part) {
+  using C = cplx<R>;
+  const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int s = find_sys(B.tstart, B.nsys, b);
+  double eacc_re = 0.0, eacc_im = 0.0;
+  const bool live = !(with_eps && B.st[s].done);
+  if (live) {
+    const OpArgs<R>& op = ops[B.op[s]];
+    const int nx = op.nx, ny = op.ny, nz = op.nz;
+    const int pl = ny * nz, nch = (pl + NT - 1) / NT;
+    const int t = b - B.tstart[s];
+    const int i = t / nch, q = (t - i * nch) * NT + (int)threadIdx.x;
+    if (q < pl) {
+      const int j = q / nz, k = q - j * nz;
+      constexpr unsigned CS = sizeof(C);
+      const C* sin = in + B.off[s];
+      DirectAcc<R> a{sin, op.k2, (unsigned)op.nn * CS, (unsigned)pl * CS, (unsigned)nz * CS,
+                     ((unsigned)i * pl + q) * CS, i, j, k, nx, ny, nz};
+      C rx, ry, rz, rp;
+      rows4<R>(i, j, k, nx, ny, nz, axis_at(op, 0, i), axis_at(op, 1, j), axis_at(op, 2, k), a,
+               rx, ry, rz, rp);
+      C* sout = out + B.off[s];
+      sto(sout, a.ob, rx);
+      sto(sout, a.nnb + a.ob, ry);
+      sto(sout, 2 * a.nnb + a.ob, rz);
+      sto(sout, 3 * a.nnb + a.ob, rp);
+      if (with_eps) {
+        const C in4[4] = {a.fv(0, 0), a.fv(1, 0), a.fv(2, 0), a.fv(3, 0)};
+        eps_acc<R>(in4, rx, ry, rz, rp, eacc_re, eacc_im);
+      }
+    }
+  }
+  if (with_eps) {
+    double v[2] = {eacc_re, eacc_im};
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+      part[(size_t)b * 2] = v[0];
+      part[(size_t)b * 2 + 1] = v[1];
+    }
+  }
+}
+
+__global__ void kb_fin_eps(BatchDev B, const double* part) {
+  const int s = blockIdx.x;
+  QmrState* st = &B.st[s];
+  if (st->done) return;
+  double tot[2];
+  b_fin_sum<2>(part, B.tstart[s], B.tstart[s + 1], tot);
+  if (threadIdx.x == 0) {
+    st->eps = {tot[0], tot[1]};
+    st->beta = cdiv(st->eps, st->del);
+    if (is_zero(st->eps) || !finite_c(st->beta)) {
+      st->status = ST_BREAK;
+      st->done = 1;
+    }
+  }
+}
+
+template <typename V>
+__global__ void kb_vtnorm(BatchDev B, const vec<V>* __restrict__ t, const vec<V>* __restrict__ v,
+                          double* part) {
+  const BItem it = b_item(B);
+  const QmrState* st = &B.st[it.s];
+  if (st->done) return;
+  const c128 beta = st->beta;
+  double a[1] = {0.0};
+  B_LOOP(q, it) a[0] += abs2(up(dn<V>(up(t[q]) - up(v[q]) * beta)));
+  block_sum<1>(a);
+  block_partial<1>(a, part);
+}
+
+__global__ void kb_fin_vtnorm(BatchDev B, const double* part) {
+  const int s = blockIdx.x;
+  QmrState* st = &B.st[s];
+  if (st->done) return;
+  double tot[1];
+  b_fin_sum<1>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) vt_scalars(st, sqrt(tot[0]));
+}
+
+template <typename V>
+__global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __restrict__ d,
+                          vec<V>* __restrict__ x, const vec<V>* __restrict__ t, vec<V>* __restrict__ s_,
+                          vec<V>* __restrict__ r, vec<V>* __restrict__ v,
+                          const vec<V>* __restrict__ invd, double* part) {
+  const BItem it = b_item(B);
+  const QmrState* st = &B.st[it.s];
+  if (st->done) return;
+  const c128 eta = st->eta, beta = st->beta;
+  const double k2 = st->k2s, rho = st->rho_np;
+  const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
+  double a[3] = {0.0, 0.0, 0.0};
+  B_LOOP(q, it)
+    update_elem<V>(q, p[q], d[q], x[q], t[q], s_[q], r[q], v[q], invd[q + io], eta, beta, k2, rho,
+                   d, x, s_, r, v, a);
+  block_sum<3>(a);
+  block_partial<3>(a, part);
+}
+
+__global__ void kb_fin_update(BatchDev B, const double* part) {
+  const int s = blockIdx.x;
+  QmrState* st = &B.st[s];
+  if (st->done) return;
+  double tot[3];
+  b_fin_sum<3>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) update_scalars(st, tot, nullptr);
+}
+
+// per-system ||b - t|| and ||b||
+template <typename V>
+__global__ void kb_resnorm(BatchDev B, const vec<V>* __restrict__ b, const vec<V>* __restrict__ t,
+                           double* part) {
+  const BItem it = b_item(B);
+  double a[2] = {0.0, 0.0};
+  B_LOOP(q, it) {
+    const c128 bq = up(b[q]);
+    a[0] += abs2(bq - up(t[q]));
+    a[1] += abs2(bq);
+  }
+  block_sum<2>(a);
+  block_partial<2>(a, part);
+}
+__global__ void kb_fin_resnorm(BatchDev B, const double* part, double* out) {
+  const int s = blockIdx.x;
+  double tot[2];
+  b_fin_sum<2>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) {
+    out[2 * s] = sqrt(tot[0]);
+    out[2 * s + 1] = sqrt(tot[1]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched mixed precision: per-system FP64 refinement around a complex64
+// inner batch.  After kb_resnorm/kb_fin_resnorm (res[2s] = ||b-Kx||,
+// res[2s+1] = ||b||), kb_inner_setup decides per system: converged (scale 0,
+// the inner right-hand side is 0 and its inner solve stops at once) or the
+// inner target; the inner rhs is (b - Kx)/||b - Kx|| in complex64.
+__global__ void kb_inner_setup(int nsys, const double* __restrict__ res, double tol,
+                               double floor_tol, const int* __restrict__ budget,
+                               double* __restrict__ scale, double* __restrict__ itol,
+                               int* __restrict__ nopen) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nsys; s += gridDim.x * blockDim.x) {
+    const double rn = res[2 * s], bn = res[2 * s + 1];
+    const bool done = !(rn >= tol * bn) || bn == 0.0;  // also NaN-safe: NaN -> not open
+    const bool open = !done && isfinite(rn) && rn > 0.0 && budget[s] > 0;
+    scale[s] = open ? 1.0 / rn : 0.0;
+    itol[s] = open ? fmax(0.5 * tol * bn / rn, floor_tol) : 1.0;
+    if (open) atomicAdd(nopen, 1);
+  }
+}
+
+__global__ void kb_inner_rhs(BatchDev B, const c128* __restrict__ b, const c128* __restrict__ t,
+                             const double* __restrict__ scale, c64* __restrict__ bi,
+                             c64* __restrict__ e) {
+  const BItem it = b_item(B);
+  const double sc = scale[it.s];
+  B_LOOP(q, it) {
+    bi[q] = dn<float>((b[q] - t[q]) * sc);
+    e[q] = c64{0.f, 0.f};
+  }
+}
+
+__global__ void kb_fin_norm_b_tol(BatchDev B, const double* part, const double* __restrict__ itol) {
+  const int s = blockIdx.x;
+  double tot[1];
+  b_fin_sum<1>(part, B.bstart[s], B.bstart[s + 1], tot);
+  if (threadIdx.x == 0) {
+    B.st[s].rhsnorm = sqrt(tot[0]);
+    B.st[s].target = itol[s] * B.st[s].rhsnorm;
+  }
+}
+
+// x += ||b - Kx|| e for the systems that were open
+__global__ void kb_refine(BatchDev B, c128* __restrict__ x, const c64* __restrict__ e,
+                          const double* __restrict__ res, const double* __restrict__ scale) {
+  const BItem it = b_item(B);
+  if (scale[it.s] == 0.0) return;
+  const double rn = res[2 * it.s];
+  B_LOOP(q, it) x[q] = x[q] + up(e[q]) * rn;
+}

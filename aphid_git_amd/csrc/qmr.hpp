@@ -168,25 +168,54 @@ __global__ void k_vtnorm(const vec<V>* __restrict__ t, const vec<V>* __restrict_
   block_sum<1>(a);
   block_partial<1>(a, part);
 }
+// step-4 scalars from rho' = ||t - beta v|| (src/aphid_qmr.f90:342-345)
+__device__ __forceinline__ void vt_scalars(QmrState* st, double rho) {
+  const c128 beta = st->beta;
+  st->rho_np = rho;
+  const double bb = beta.re * beta.re + beta.im * beta.im;
+  st->theta_n = rho * rho / st->c_nm / bb;
+  st->c_n = 1.0 / (1.0 + st->theta_n);
+  const c128 num = st->c_n * (st->rho_n * (-st->eta));
+  st->eta = cdiv(num, st->c_nm * beta);
+  st->k2s = st->theta_nm * st->c_n;
+  if (!(rho > 0.0) || !isfinite(rho) || !finite_c(st->eta)) {
+    st->status = ST_BREAK;
+    st->done = 1;
+  }
+}
+
 __global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
   if (st->done) return;
   double tot[1];
   finalize_sum<1>(part, nb, tot);
-  if (threadIdx.x == 0) {
-    const c128 beta = st->beta;
-    const double rho = sqrt(tot[0]);
-    st->rho_np = rho;
-    const double bb = beta.re * beta.re + beta.im * beta.im;
-    st->theta_n = rho * rho / st->c_nm / bb;
-    st->c_n = 1.0 / (1.0 + st->theta_n);
-    const c128 num = st->c_n * (st->rho_n * (-st->eta));
-    st->eta = cdiv(num, st->c_nm * beta);
-    st->k2s = st->theta_nm * st->c_n;
-    if (!(rho > 0.0) || !isfinite(rho) || !finite_c(st->eta)) {
-      st->status = ST_BREAK;
-      st->done = 1;
-    }
-  }
+  if (threadIdx.x == 0) vt_scalars(st, sqrt(tot[0]));
+}
+
+// one element of step D: d, x, s, r, v updates; accumulates ||r||^2 and
+// v^T M^-1 v (src/aphid_qmr.f90:347-369, 306-311)
+template <typename V>
+__device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_,
+                                            vec<V> sq, vec<V> rq, vec<V> vq, vec<V> iq, c128 eta,
+                                            c128 beta, double k2, double rho,
+                                            vec<V>* __restrict__ d, vec<V>* __restrict__ x,
+                                            vec<V>* __restrict__ s, vec<V>* __restrict__ r,
+                                            vec<V>* __restrict__ v, double (&a)[3]) {
+  const c128 tq = up(tq_);
+  const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
+  d[q] = ddv;
+  x[q] = dn<V>(up(xq) + up(ddv));
+  const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
+  s[q] = ssv;
+  const vec<V> rrv = dn<V>(up(rq) - up(ssv));
+  r[q] = rrv;
+  const vec<V> vt = dn<V>(tq - up(vq) * beta);
+  const vec<V> vv = dn<V>(divr(up(vt), rho));
+  v[q] = vv;
+  a[0] += abs2(up(rrv));
+  const c128 vd = up(vv);
+  const c128 pr = vd * (vd * up(iq));
+  a[1] += pr.re;
+  a[2] += pr.im;
 }
 
 // D: d = eta p + k2 d; x += d; s = eta t + k2 s; r -= s; v = (t - beta v)/rho';
@@ -203,22 +232,7 @@ __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, v
   double a[3] = {0.0, 0.0, 0.0};
   auto body = [&](long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_, vec<V> sq, vec<V> rq,
                   vec<V> vq, vec<V> iq) {
-    const c128 tq = up(tq_);
-    const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
-    d[q] = ddv;
-    x[q] = dn<V>(up(xq) + up(ddv));
-    const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
-    s[q] = ssv;
-    const vec<V> rrv = dn<V>(up(rq) - up(ssv));
-    r[q] = rrv;
-    const vec<V> vt = dn<V>(tq - up(vq) * beta);
-    const vec<V> vv = dn<V>(divr(up(vt), rho));
-    v[q] = vv;
-    a[0] += abs2(up(rrv));
-    const c128 vd = up(vv);
-    const c128 pr = vd * (vd * up(iq));
-    a[1] += pr.re;
-    a[2] += pr.im;
+    update_elem<V>(q, pq, dq, xq, tq_, sq, rq, vq, iq, eta, beta, k2, rho, d, x, s, r, v, a);
   };
   UNROLLED_LOOP(2, q, stride, n) {
     vec<V> P_[2], D_[2], X_[2], T_[2], S_[2], R_[2], Vv[2], I_[2];
@@ -235,44 +249,47 @@ __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, v
   block_sum<3>(a);
   block_partial<3>(a, part);
 }
+// ||r||, history, stop test and counter (src/aphid_qmr.f90:371-390), then
+// the loop-entry scalars of the next iteration (:300-313)
+__device__ __forceinline__ void update_scalars(QmrState* st, const double (&tot)[3], double* hist) {
+  const double rnorm = sqrt(tot[0]);
+  st->rnorm = rnorm;
+  const int it = st->iter;
+  if (hist && it >= 1 && it <= st->hist_cap) hist[it - 1] = rnorm;
+  if (rnorm < st->target) {
+    st->status = ST_CONV;
+    st->done = 1;
+  } else if (!isfinite(rnorm)) {
+    st->status = ST_BREAK;
+    st->done = 1;
+  } else if (it <= st->max_it) {
+    st->iter = it + 1;
+    if (it + 1 > st->max_it) {
+      st->status = ST_MAXIT;
+      st->done = 1;
+    }
+  } else {
+    st->status = ST_MAXIT;
+    st->done = 1;
+  }
+  if (!st->done) {
+    st->c_nm = st->c_n;
+    st->rho_n = st->rho_np;
+    st->theta_nm = st->theta_n;
+    st->del = {tot[1], tot[2]};
+    st->k1 = cdiv(st->rho_n * st->del, st->eps);
+    if (is_zero(st->del) || !finite_c(st->k1)) {
+      st->status = ST_BREAK;
+      st->done = 1;
+    }
+  }
+}
+
 __global__ void k_fin_update(const double* part, int nb, QmrState* st, double* hist) {
   if (st->done) return;
   double tot[3];
   finalize_sum<3>(part, nb, tot);
-  if (threadIdx.x == 0) {
-    const double rnorm = sqrt(tot[0]);
-    st->rnorm = rnorm;
-    const int it = st->iter;
-    if (hist && it >= 1 && it <= st->hist_cap) hist[it - 1] = rnorm;
-    // stop test and counter exactly as src/aphid_qmr.f90:386-390
-    if (rnorm < st->target) {
-      st->status = ST_CONV;
-      st->done = 1;
-    } else if (!isfinite(rnorm)) {
-      st->status = ST_BREAK;
-      st->done = 1;
-    } else if (it <= st->max_it) {
-      st->iter = it + 1;
-      if (it + 1 > st->max_it) {
-        st->status = ST_MAXIT;
-        st->done = 1;
-      }
-    } else {
-      st->status = ST_MAXIT;
-      st->done = 1;
-    }
-    if (!st->done) {  // start of the next iteration (:300-313)
-      st->c_nm = st->c_n;
-      st->rho_n = st->rho_np;
-      st->theta_nm = st->theta_n;
-      st->del = {tot[1], tot[2]};
-      st->k1 = cdiv(st->rho_n * st->del, st->eps);
-      if (is_zero(st->del) || !finite_c(st->k1)) {
-        st->status = ST_BREAK;
-        st->done = 1;
-      }
-    }
-  }
+  if (threadIdx.x == 0) update_scalars(st, tot, hist);
 }
 
 // ||b - t||^2 / ||b||^2 for the true residual (t = K x)

@@ -274,8 +274,94 @@ class Solver:
         return out
 
 
+class SysReport(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("rnorm", C.c_double),
+                ("rhsnorm", C.c_double), ("true_rel_res", C.c_double)]
+
+
+class BatchSolver:
+    """Many independent systems in one launch per QMR phase (the on-GPU
+    replacement of the reference's MPI farm, src/aphid_parallel.f90)."""
+
+    def __init__(self, device: int = -1):
+        self._b = C.c_void_p()
+        _check(lib().aphid_batch_create(C.byref(self._b), C.c_int(device)), "aphid_batch_create")
+        self.op_shapes: List[Tuple[int, int, int]] = []
+        self.sys_ops: List[int] = []
+
+    def close(self):
+        if self._b:
+            lib().aphid_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_operator(self, xn, yn, zn, model: str, omega: float, cells: dict) -> int:
+        xn, yn, zn = (np.ascontiguousarray(a, np.float64) for a in (xn, yn, zn))
+        nx, ny, nz = len(xn), len(yn), len(zn)
+        cs = (nx - 1, ny - 1, nz - 1)
+        blocks = {"R": ("sigma", None, None, None), "C": ("sigma", "eps_rel", None, None),
+                  "D": ("sigma", "eta", "tau", "cee"), "I": (None, "eps_rel", None, None)}[model]
+        arrs = []
+        for n in blocks:
+            a = None if n is None else np.asfortranarray(cells[n], dtype=np.float32)
+            if a is not None and a.shape != cs:
+                raise ValueError(f"cell block {n}: expected {cs}, got {a.shape}")
+            arrs.append(a)
+        rc = _check(lib().aphid_batch_add_operator(self._b, nx, ny, nz, _p(xn), _p(yn), _p(zn),
+                                                    C.c_char(model.encode()), C.c_double(omega),
+                                                    *[_p(a) for a in arrs]),
+                    "aphid_batch_add_operator")
+        self.op_shapes.append((nx, ny, nz))
+        return rc
+
+    def add_case_operator(self, case) -> int:
+        return self.add_operator(case.xn, case.yn, case.zn, case.model, case.omega, case.cells)
+
+    def add_system(self, op: int, b, x0=None) -> int:
+        shp = comp_shapes(*self.op_shapes[op])
+        b = [_c128(a, s) for a, s in zip(b, shp)]
+        x0a = [None] * 4 if x0 is None else [_c128(a, s) for a, s in zip(x0, shp)]
+        rc = _check(lib().aphid_batch_add_system(self._b, op, *[_p(a) for a in b],
+                                                  *[_p(a) for a in x0a]), "aphid_batch_add_system")
+        self.sys_ops.append(op)
+        return rc
+
+    def solve(self, tol: float, max_it: int, flags: int = 0):
+        ns = len(self.sys_ops)
+        reps = (SysReport * ns)()
+        secs = C.c_double()
+        _check(lib().aphid_batch_solve(self._b, C.c_double(tol), C.c_int(max_it), C.c_uint(flags),
+                                       reps, C.byref(secs)), "aphid_batch_solve")
+        return [dict(iters=r.iters, status=r.status, rnorm=r.rnorm, rhsnorm=r.rhsnorm,
+                     true_rel_res=r.true_rel_res) for r in reps], secs.value
+
+    def solution(self, sys: int):
+        out = [np.zeros(s, np.complex128, order="F")
+               for s in comp_shapes(*self.op_shapes[self.sys_ops[sys]])]
+        _check(lib().aphid_batch_get_solution(self._b, sys, *[_p(a) for a in out]),
+               "aphid_batch_get_solution")
+        return out
+
+    def set_guess(self, sys: int, from_sys: int):
+        _check(lib().aphid_batch_set_guess(self._b, sys, from_sys), "aphid_batch_set_guess")
+
+
 EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_set_mesh",
            "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
            "aphid_get_diag", "aphid_apply", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
            "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
-           "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy"]
+           "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy",
+           "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",
+           "aphid_batch_add_system", "aphid_batch_solve", "aphid_batch_get_solution",
+           "aphid_batch_set_guess"]

@@ -176,6 +176,54 @@ int aphid_time_iteration(aphid_ctx *ctx, int precision, int reps, double *secs);
  * calibrates the roofline and the hardware traffic counters. */
 int aphid_time_copy(aphid_ctx *ctx, double bytes, int reps, double *seconds);
 
+/* ---------------------------------------------------------------------------
+ * Batched solve: many independent systems (frequencies x sources x meshes of
+ * uneven size) in one launch per phase of the QMR iteration.  This replaces
+ * the reference's MPI process farm (src/aphid_parallel.f90:75-201, one
+ * (frequency, source) system per rank) on a single GPU.  Systems that share a
+ * mesh and frequency share one operator.  Each system stops on its own test
+ * (||r|| < tol ||b||, src/aphid_qmr.f90:386); converged systems drop out of
+ * every later launch.  Host arrays as in the single-system API. */
+typedef struct aphid_batch aphid_batch;
+
+typedef struct {
+  int iters;           /* the reference's `iter` for this system */
+  int status;          /* APHID_OK / APHID_MAXIT / APHID_BREAKDOWN */
+  double rnorm;        /* final recurrence residual norm */
+  double rhsnorm;      /* ||b|| */
+  double true_rel_res; /* ||b - K x|| / ||b|| in complex128 */
+} aphid_sys_report;
+
+int aphid_batch_create(aphid_batch **batch, int device);
+int aphid_batch_destroy(aphid_batch *batch);
+/* An operator: mesh + cell parameters at angular frequency omega (as
+ * aphid_set_mesh + aphid_set_cells).  Returns its index (>= 0). */
+int aphid_batch_add_operator(aphid_batch *batch, int nx, int ny, int nz,
+                             const double *xn, const double *yn,
+                             const double *zn, char model, double omega,
+                             const float *a, const float *b, const float *c,
+                             const float *d);
+/* A system on operator `op`: right-hand side and an optional initial guess
+ * (NULL = 0).  Add every operator before the first system and every system
+ * before the first solve.  Returns the system index (>= 0). */
+int aphid_batch_add_system(aphid_batch *batch, int op, const double *bx_c,
+                           const double *by_c, const double *bz_c,
+                           const double *bp_c, const double *x0_x,
+                           const double *x0_y, const double *x0_z,
+                           const double *x0_p);
+/* Solve every system.  reports (optional) has one entry per system; returns
+ * the worst status.  seconds (optional): device time of the solve.  With
+ * APHID_F_X0 the current solutions are the initial guesses (warm restart,
+ * e.g. after aphid_batch_set_guess). */
+int aphid_batch_solve(aphid_batch *batch, double tol, int max_it,
+                      unsigned flags, aphid_sys_report *reports,
+                      double *seconds);
+int aphid_batch_get_solution(aphid_batch *batch, int sys, double *x_x,
+                             double *x_y, double *x_z, double *x_p);
+/* Use system from_sys's current solution as the initial guess of system
+ * sys (same mesh), e.g. the previous frequency of a sweep. */
+int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
+
 #ifdef __cplusplus
 }
 #endif
