@@ -31,13 +31,27 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+SOURCES = ("aphid_gpu.hip", "unet.hip")
+
+
 def build_hip(verbose_resources: bool = False) -> str:
+    """Each translation unit to an object (in parallel), then one shared library."""
     out = os.path.join(HERE, "libaphid_gpu.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "aphid_gpu.hip")]
-    if verbose_resources:
-        cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
-    _run(cmd)
+    bdir = os.path.join(HERE, "_obj")
+    os.makedirs(bdir, exist_ok=True)
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose_resources:
+            cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+    if any(p.wait() != 0 for p in procs):
+        raise RuntimeError("hipcc failed")
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
     return out
 

@@ -80,4 +80,4 @@ struct QmrState {
 
 enum : int { ST_CONV = 0, ST_MAXIT = 1, ST_BREAK = 2 };
 
-__device__ __forceinline__ bool finite_c(c128 a) { return isfinite(a.re) && isfinite(a.im); }
+__host__ __device__ __forceinline__ bool finite_c(c128 a) { return std::isfinite(a.re) && std::isfinite(a.im); }

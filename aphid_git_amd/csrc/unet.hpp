@@ -1,0 +1,477 @@
+// unet.hpp -- kernels of the warm-start network: 3-D convolutions as
+// implicit GEMMs on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
+// accumulate), their input and weight gradients, 2x pooling / upsampling,
+// the loss and the Adam update.
+//
+// Tensors are channels-last on a cell grid with z fastest:
+//   t[((x*Y + y)*Z + z)*ld + off + c]
+// so that the 8 channels of one (voxel, tap) pair are one 16-byte load.
+// Activations are bf16 (stored as raw 16-bit patterns), activation
+// gradients FP32, parameters FP32 with bf16 copies for the matrix cores.
+//
+// Convolution weights in kernel layout: wk[co][kk], kk = tap*CIN + ci
+// (K-major per output channel), rows padded to a multiple of 16 and K to a
+// multiple of 32 with zeros.  tap = (dx+1)*9 + (dy+1)*3 + (dz+1) for a 3x3x3
+// kernel (cross-correlation, zero padding), tap = 0 for a 1x1 kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16_t;  // raw bf16 bit pattern
+
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 ld8(const bf16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+__device__ __forceinline__ bf16x8 zero8() {
+  const u32x4 z = {0u, 0u, 0u, 0u};
+  return __builtin_bit_cast(bf16x8, z);
+}
+
+constexpr int cpad(int n, int m) { return (n + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// Convolution, forward and input-gradient (the same implicit GEMM):
+//   out[v][co] = sum_{tap,ci} w[co][tap*CIN+ci] * in[v + off(tap)][ci]
+// MODE 0: + bias, ReLU, bf16 out.  MODE 1: + bias, FP32 out.
+// MODE 2: FP32 out, no bias (input gradient with flipped, transposed weights).
+struct ConvArgs {
+  int X, Y, Z, V;
+  const bf16_t* in;
+  int in_ld, in_off;
+  const bf16_t* w;     // [cpad(COUT,16)][cpad(NT*CIN,32)]
+  const float* bias;   // [COUT] (MODE 0/1)
+  void* out;
+  int out_ld, out_off;
+};
+
+template <int KS, int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(256) void kc_conv(ConvArgs a) {
+  constexpr int NT = KS * KS * KS;
+  constexpr int KPAD = cpad(NT * CIN, 32);
+  constexpr int NB = (COUT + 15) / 16;
+  constexpr int MT = NB >= 8 ? 1 : 8 / NB;  // 16-voxel tiles per wave
+  static_assert(CIN % 8 == 0, "CIN must be a multiple of 8");
+  const int lane = threadIdx.x % 64, r = lane % 16, g = lane / 16;
+  const int wave = blockIdx.x * 4 + threadIdx.x / 64;
+  const int YZ = a.Y * a.Z;
+  const int v0 = wave * MT * 16;
+  if (v0 >= a.V) return;
+
+  int vx[MT], vy[MT], vz[MT];
+  bool vv[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int v = v0 + 16 * t + r;
+    vv[t] = v < a.V;
+    const int vc = vv[t] ? v : 0;
+    vx[t] = vc / YZ;
+    const int rem = vc - vx[t] * YZ;
+    vy[t] = rem / a.Z;
+    vz[t] = rem - vy[t] * a.Z;
+  }
+  f32x4 acc[MT][NB];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int ks = 0; ks < KPAD / 32; ++ks) {
+    const int kk = ks * 32 + 8 * g;
+    const int tap = kk / CIN, ci = kk - tap * CIN;
+    int dx = 0, dy = 0, dz = 0;
+    if (KS == 3) {
+      dx = tap / 9 - 1;
+      dy = (tap / 3) % 3 - 1;
+      dz = tap % 3 - 1;
+    }
+    const bool tv = tap < NT;
+    bf16x8 bw[NB];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) bw[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + kk);
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      const int x = vx[t] + dx, y = vy[t] + dy, z = vz[t] + dz;
+      const bool ok = tv && vv[t] && x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z;
+      const int vn = (x * a.Y + y) * a.Z + z;
+      const bf16x8 av = ok ? ld8(a.in + (size_t)vn * a.in_ld + a.in_off + ci) : zero8();
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[n], acc[t][n], 0, 0, 0);
+    }
+  }
+  // D[i][j]: lane holds rows i = 4g+e (voxels), column j = r (output channel)
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      const int co = 16 * n + r;
+      if (co >= COUT) continue;
+      const float b = MODE == 2 ? 0.f : a.bias[co];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int v = v0 + 16 * t + 4 * g + e;
+        if (v >= a.V) continue;
+        float val = acc[t][n][e] + b;
+        const size_t o = (size_t)v * a.out_ld + a.out_off + co;
+        if (MODE == 0) {
+          val = fmaxf(val, 0.f);
+          reinterpret_cast<bf16_t*>(a.out)[o] = f2bf(val);
+        } else {
+          reinterpret_cast<float*>(a.out)[o] = val;
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dw[co][kk] = sum_v gy[v][co] * in[v + off(tap(kk))][ci(kk)]
+// A GEMM with K running over voxels.  A block owns 256 columns kk (4 waves x
+// 4 blocks of 16) and one voxel split; each 32-voxel step stages gy and the
+// gathered inputs transposed ([row][voxel]) in LDS, so that every lane reads
+// its 8 consecutive voxels as one 16-byte LDS load.  Per-split partials are
+// summed in a fixed order by kc_dw_reduce.
+struct DwArgs {
+  int X, Y, Z, V;
+  const bf16_t* gy;  // [V][COUT]
+  const bf16_t* in;
+  int in_ld, in_off;
+  float* part;       // [nsplit][cpad(COUT,16)][KPAD]
+  int vper;          // voxels per split, a multiple of 32
+};
+
+constexpr int kLdsPitch = 40;  // bf16 per LDS row (32 voxels + 8 pad)
+
+template <int KS, int CIN, int COUT>
+__global__ __launch_bounds__(256) void kc_dw(DwArgs a) {
+  constexpr int NT = KS * KS * KS;
+  constexpr int KTOT = NT * CIN;
+  constexpr int KPAD = cpad(KTOT, 32);
+  constexpr int COUTP = cpad(COUT, 16);
+  constexpr int MB = COUTP / 16;
+  __shared__ __attribute__((aligned(16))) bf16_t gs[COUTP][kLdsPitch];
+  __shared__ __attribute__((aligned(16))) bf16_t xs[256][kLdsPitch];
+  const int tid = threadIdx.x, lane = tid % 64, r = lane % 16, g = lane / 16, w = tid / 64;
+  const int kbase = blockIdx.x * 256;
+  const int s = blockIdx.y;
+  const int vb0 = s * a.vper;
+  const int vb1 = min(a.V, vb0 + a.vper);
+  const int YZ = a.Y * a.Z;
+
+  f32x4 acc[MB][4];
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int vb = vb0; vb < vb1; vb += 32) {
+    // stage gy: 32 voxels x COUTP/8 pieces of 8 channels
+    if (tid < 32 * (COUTP / 8)) {
+      const int vl = tid % 32, pc = tid / 32, v = vb + vl;
+      bf16_t h[8];
+      if (v < vb1 && 8 * pc < COUT) {
+        const uint4 q = *reinterpret_cast<const uint4*>(a.gy + (size_t)v * COUT + 8 * pc);
+        const unsigned qq[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          h[2 * e] = (bf16_t)(qq[e] & 0xffffu);
+          h[2 * e + 1] = (bf16_t)(qq[e] >> 16);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) h[e] = 0;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) gs[8 * pc + e][vl] = h[e];
+    }
+    // stage the gathered inputs: 32 pieces of 8 columns x 32 voxels
+#pragma unroll
+    for (int q4 = 0; q4 < 4; ++q4) {
+      const int idx = tid + 256 * q4, vl = idx % 32, pc = idx / 32;
+      const int kk0 = kbase + 8 * pc, v = vb + vl;
+      bf16_t h[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) h[e] = 0;
+      if (kk0 < KTOT && v < vb1) {
+        const int tap = kk0 / CIN, ci = kk0 - tap * CIN;
+        int dx = 0, dy = 0, dz = 0;
+        if (KS == 3) {
+          dx = tap / 9 - 1;
+          dy = (tap / 3) % 3 - 1;
+          dz = tap % 3 - 1;
+        }
+        const int x0 = v / YZ, rem = v - x0 * YZ, y0 = rem / a.Z, z0 = rem - y0 * a.Z;
+        const int x = x0 + dx, y = y0 + dy, z = z0 + dz;
+        if (x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z) {
+          const int vn = (x * a.Y + y) * a.Z + z;
+          const uint4 q =
+              *reinterpret_cast<const uint4*>(a.in + (size_t)vn * a.in_ld + a.in_off + ci);
+          const unsigned qq[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            h[2 * e] = (bf16_t)(qq[e] & 0xffffu);
+            h[2 * e + 1] = (bf16_t)(qq[e] >> 16);
+          }
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xs[8 * pc + e][vl] = h[e];
+    }
+    __syncthreads();
+    bf16x8 bx[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) bx[n] = ld8(&xs[w * 64 + 16 * n + r][8 * g]);
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const bf16x8 ag = ld8(&gs[16 * m + r][8 * g]);
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ag, bx[n], acc[m][n], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* P = a.part + (size_t)s * COUTP * KPAD;
+#pragma unroll
+  for (int m = 0; m < MB; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int kk = kbase + w * 64 + 16 * n + r;
+      if (kk >= KPAD) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) P[(size_t)(16 * m + 4 * g + e) * KPAD + kk] = acc[m][n][e];
+    }
+}
+
+// grad[(co*cin_real + ci)*NT + tap] += sum_s part[s][co][tap*cinp + ci]
+__global__ void kc_dw_reduce(const float* __restrict__ part, int nsplit, int coutp, int kpad,
+                             int cout, int cin_real, int cinp, int nt, float* __restrict__ grad) {
+  const int n = cout * cin_real * nt;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int tap = q % nt, ci = (q / nt) % cin_real, co = q / (nt * cin_real);
+    const size_t e = (size_t)co * kpad + tap * cinp + ci;
+    float acc = 0.f;
+    for (int s = 0; s < nsplit; ++s) acc += part[(size_t)s * coutp * kpad + e];
+    grad[q] += acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ReLU backward and bias gradient:
+//   gy[v][c] = bf16(dy[v][c] * (y[v][c] > 0))   (y == nullptr: no ReLU)
+//   part[block][c] = sum over the block's voxels of gy (FP32, pre-rounding)
+// C divides 256, so every thread of the grid-stride loop keeps one channel.
+__global__ __launch_bounds__(256) void kc_relu_bwd(int V, int C, const float* __restrict__ dy,
+                                                   int dy_ld, int dy_off, const bf16_t* __restrict__ y,
+                                                   int y_ld, int y_off, bf16_t* __restrict__ gy,
+                                                   float* __restrict__ part) {
+  __shared__ float red[256];
+  const int tid = threadIdx.x, c = tid % C;
+  float s = 0.f;
+  const long long n = (long long)V * C;
+  for (long long q = (long long)blockIdx.x * 256 + tid; q < n; q += (long long)gridDim.x * 256) {
+    const long long v = q / C;
+    float d = dy[v * dy_ld + dy_off + c];
+    if (y && !(bf2f(y[v * y_ld + y_off + c]) > 0.f)) d = 0.f;
+    const bf16_t h = f2bf(d);
+    gy[q] = h;
+    s += bf2f(h);
+  }
+  red[tid] = s;
+  __syncthreads();
+  if (tid < C) {
+    float t = 0.f;
+    for (int u = tid; u < 256; u += C) t += red[u];
+    part[(size_t)blockIdx.x * C + tid] = t;
+  }
+}
+
+// db[c] += sum_b part[b][c]  (fixed order)
+__global__ void kc_bias_reduce(const float* __restrict__ part, int nb, int C, float* __restrict__ db) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float t = 0.f;
+  for (int b = 0; b < nb; ++b) t += part[(size_t)b * C + c];
+  db[c] += t;
+}
+
+// ---------------------------------------------------------------------------
+// 2x average pooling and nearest upsampling between a fine grid (X,Y,Z)
+// and its coarse grid (X/2,Y/2,Z/2).
+__device__ __forceinline__ int fine_of(int xc, int yc, int zc, int a, int Y, int Z) {
+  return ((2 * xc + (a >> 2)) * Y + (2 * yc + ((a >> 1) & 1))) * Z + (2 * zc + (a & 1));
+}
+
+__global__ void kc_pool(int X, int Y, int Z, int C, const bf16_t* __restrict__ in, int in_ld,
+                        int in_off, bf16_t* __restrict__ out) {
+  const int Xc = X / 2, Yc = Y / 2, Zc = Z / 2;
+  const long long n = (long long)Xc * Yc * Zc * C;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const int vc = (int)(q / C);
+    const int zc = vc % Zc, yc = (vc / Zc) % Yc, xc = vc / (Zc * Yc);
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s += bf2f(in[(size_t)fine_of(xc, yc, zc, a, Y, Z) * in_ld + in_off + c]);
+    out[q] = f2bf(s * 0.125f);
+  }
+}
+
+// d_in[fine][c] += d_out[coarse][c] / 8
+__global__ void kc_pool_bwd(int X, int Y, int Z, int C, const float* __restrict__ dout,
+                            float* __restrict__ din, int din_ld, int din_off) {
+  const long long n = (long long)X * Y * Z * C;
+  const int Yc = Y / 2, Zc = Z / 2;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const int v = (int)(q / C);
+    const int z = v % Z, y = (v / Z) % Y, x = v / (Z * Y);
+    const int vc = ((x / 2) * Yc + y / 2) * Zc + z / 2;
+    din[(size_t)v * din_ld + din_off + c] += 0.125f * dout[(size_t)vc * C + c];
+  }
+}
+
+__global__ void kc_up(int X, int Y, int Z, int C, const bf16_t* __restrict__ in,
+                      bf16_t* __restrict__ out, int out_ld, int out_off) {
+  const long long n = (long long)X * Y * Z * C;
+  const int Yc = Y / 2, Zc = Z / 2;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const int v = (int)(q / C);
+    const int z = v % Z, y = (v / Z) % Y, x = v / (Z * Y);
+    const int vc = ((x / 2) * Yc + y / 2) * Zc + z / 2;
+    out[(size_t)v * out_ld + out_off + c] = in[(size_t)vc * C + c];
+  }
+}
+
+// d_in[coarse][c] = sum of the 8 children's d_out
+__global__ void kc_up_bwd(int X, int Y, int Z, int C, const float* __restrict__ dout, int dout_ld,
+                          int dout_off, float* __restrict__ din) {
+  const int Xc = X / 2, Yc = Y / 2, Zc = Z / 2;
+  const long long n = (long long)Xc * Yc * Zc * C;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(q % C);
+    const int vc = (int)(q / C);
+    const int zc = vc % Zc, yc = (vc / Zc) % Yc, xc = vc / (Zc * Yc);
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s += dout[(size_t)fine_of(xc, yc, zc, a, Y, Z) * dout_ld + dout_off + c];
+    din[q] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mean squared error over V voxels x 8 channels; dout = scale * 2 (out - t)
+// with scale = 1 / (8 V batch); part[block] = sum (out - t)^2 (FP64).
+__global__ __launch_bounds__(256) void kc_loss(long long n, const float* __restrict__ out,
+                                               const bf16_t* __restrict__ tgt, float scale,
+                                               float* __restrict__ dout, double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    const float d = out[q] - bf2f(tgt[q]);
+    dout[q] = 2.f * scale * d;
+    s += (double)d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void kc_loss_reduce(const double* __restrict__ part, int nb, double scale,
+                               double* __restrict__ acc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += part[b];
+  *acc += t * scale;
+}
+
+// ---------------------------------------------------------------------------
+// Parameters, gradients and data movement
+__global__ void kc_fill(float* __restrict__ p, long long n, float v) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+
+// Adam (Kingma & Ba 2015) on the flat FP32 parameters; g is the summed
+// gradient of the batch, gscale = 1/batch; bc1 = 1 - b1^t, bc2 = 1 - b2^t.
+__global__ void kc_adam(long long n, float* __restrict__ p, const float* __restrict__ g,
+                        float* __restrict__ m, float* __restrict__ v, float lr, float b1, float b2,
+                        float eps, float bc1, float bc2, float gscale) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const float gq = g[q] * gscale;
+    const float mq = b1 * m[q] + (1.f - b1) * gq;
+    const float vq = b2 * v[q] + (1.f - b2) * gq * gq;
+    m[q] = mq;
+    v[q] = vq;
+    p[q] -= lr * (mq / bc1) / (sqrtf(vq / bc2) + eps);
+  }
+}
+
+// Kernel-layout bf16 weights of one layer from the flat FP32 parameters
+// w[(co*cin_real + ci)*nt + tap]:
+//   forward  wf[co][tap*cinp + ci]             rows coutp, K kpad
+//   backward wt[ci][tap'*cout + co] = w[co][ci][nt-1-tap']   rows cinp_t, K kpad_t
+__global__ void kc_wconv(const float* __restrict__ w, int cout, int cin_real, int cinp, int nt,
+                         int coutp, int kpad, bf16_t* __restrict__ wf, int cinp_t, int kpad_t,
+                         bf16_t* __restrict__ wt) {
+  const int nf = coutp * kpad, nb = cinp_t * kpad_t;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf + nb; q += gridDim.x * blockDim.x) {
+    if (q < nf) {
+      const int co = q / kpad, kk = q % kpad, tap = kk / cinp, ci = kk % cinp;
+      float val = 0.f;
+      if (co < cout && tap < nt && ci < cin_real) val = w[((size_t)co * cin_real + ci) * nt + tap];
+      wf[q] = f2bf(val);
+    } else if (wt) {
+      const int e = q - nf;
+      const int ci = e / kpad_t, kk = e % kpad_t, tp = kk / cout, co = kk % cout;
+      float val = 0.f;
+      if (ci < cin_real && tp < nt) val = w[((size_t)co * cin_real + ci) * nt + (nt - 1 - tp)];
+      wt[e] = f2bf(val);
+    }
+  }
+}
+
+// f32 -> bf16 with channel padding: dst[v][0..cdst) = src[v][0..csrc), 0 beyond
+__global__ void kc_pack_bf16(long long V, int csrc, int cdst, const float* __restrict__ src,
+                             bf16_t* __restrict__ dst) {
+  const long long n = V * cdst;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long v = q / cdst;
+    const int c = (int)(q % cdst);
+    dst[q] = c < csrc ? f2bf(src[v * csrc + c]) : (bf16_t)0;
+  }
+}
+
+// dst[v][0..cdst) = src[v][0..csrc) (bf16), zero channels beyond csrc
+__global__ void kc_widen(long long V, int csrc, int cdst, const bf16_t* __restrict__ src,
+                         bf16_t* __restrict__ dst) {
+  const long long n = V * cdst;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long v = q / cdst;
+    const int c = (int)(q % cdst);
+    dst[q] = c < csrc ? src[v * csrc + c] : (bf16_t)0;
+  }
+}

@@ -25,6 +25,8 @@ _lib = None
 F_X0 = 1
 F_NO_TRUE_RES = 2
 F_MIXED = 4
+WS_PREV = 1
+WS_LSQ = 2
 STATUS = {0: "converged", 1: "max_it", 2: "breakdown"}
 
 
@@ -47,6 +49,13 @@ def lib():
             raise AphidError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
         _lib = C.CDLL(LIB_PATH)
         _lib.aphid_last_error.restype = C.c_char_p
+        _lib.aphid_unet_create.restype = C.c_void_p
+        _lib.aphid_unet_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]
+        _lib.aphid_unet_destroy.argtypes = [C.c_void_p]
+        _lib.aphid_unet_param_count.restype = C.c_longlong
+        _lib.aphid_unet_param_count.argtypes = [C.c_void_p]
+        _lib.aphid_unet_forward_flops.restype = C.c_double
+        _lib.aphid_unet_forward_flops.argtypes = [C.c_void_p]
     return _lib
 
 
@@ -239,6 +248,19 @@ class Solver:
                            rep.seconds_qmr, rep.seconds_total, rep.outer_iters,
                            hist[:rep.hist_len].copy())
 
+    def warm_start(self, net: Optional["UNet"] = None, features=None, prev: bool = False,
+                   lsq: bool = False) -> float:
+        """Initial guess for solve_resident(..., flags=F_X0) from the network
+        and/or the current solution (prev), amplitudes by the Galerkin
+        condition (or least squares on the residual); returns
+        ||b - K x0|| / ||b||."""
+        r0 = C.c_double()
+        f = None if features is None else np.ascontiguousarray(features, np.float32)
+        flags = (WS_PREV if prev else 0) | (WS_LSQ if lsq else 0)
+        _check(lib().aphid_warm_start(self._ctx, None if net is None else C.c_void_p(net._n),
+                                      _p(f), C.c_uint(flags), C.byref(r0)), "aphid_warm_start")
+        return r0.value
+
     def fetch_solution(self):
         out = self._zeros4()
         _check(lib().aphid_fetch_solution(self._ctx, *[_p(a) for a in out]),
@@ -357,6 +379,102 @@ class BatchSolver:
         _check(lib().aphid_batch_set_guess(self._b, sys, from_sys), "aphid_batch_set_guess")
 
 
+class UNet:
+    """The warm-start network (aphid_unet_*): a 3-level 3-D U-Net on a cell
+    grid of (nx, ny, nz) cells, trained and evaluated on the GPU."""
+
+    # buffer indices of aphid_unet_activation (the network's tensor list)
+    BUFFERS = ["in", "a1", "cat1", "p1", "a2", "cat2", "p2", "a3", "b3", "r3",
+               "d2a", "d2b", "r2", "d1a", "d1b", "out"]
+    CHANNELS = [8, 16, 32, 16, 32, 64, 32, 64, 64, 32, 32, 32, 16, 16, 16, 8]
+    LEVELS = [0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 1, 1, 1, 0, 0, 0]
+
+    def __init__(self, cells: Tuple[int, int, int], seed: int = 0, device: int = 0):
+        self.cells = tuple(int(c) for c in cells)
+        n = lib().aphid_unet_create(device, *self.cells, C.c_ulonglong(seed))
+        if not n:
+            raise AphidError(f"aphid_unet_create: {lib().aphid_last_error().decode()}")
+        self._n = n
+        self.n_params = int(lib().aphid_unet_param_count(C.c_void_p(n)))
+        self.V = int(np.prod(self.cells))
+
+    def close(self):
+        if self._n:
+            lib().aphid_unet_destroy(C.c_void_p(self._n))
+            self._n = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def forward_flops(self) -> float:
+        return float(lib().aphid_unet_forward_flops(C.c_void_p(self._n)))
+
+    def _f(self, a, ch):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.size != self.V * ch:
+            raise ValueError(f"expected {self.V}x{ch} values, got {a.shape}")
+        return a
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().aphid_unet_get_params(C.c_void_p(self._n), _p(out)), "aphid_unet_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("wrong parameter count")
+        _check(lib().aphid_unet_set_params(C.c_void_p(self._n), _p(p)), "aphid_unet_set_params")
+
+    def forward(self, features) -> np.ndarray:
+        out = np.zeros((self.V, 8), np.float32)
+        _check(lib().aphid_unet_forward(C.c_void_p(self._n), _p(self._f(features, 3)), _p(out)),
+               "aphid_unet_forward")
+        return out
+
+    def loss_grad(self, features, target):
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _check(lib().aphid_unet_loss_grad(C.c_void_p(self._n), _p(self._f(features, 3)),
+                                          _p(self._f(target, 8)), C.byref(loss), _p(g)),
+               "aphid_unet_loss_grad")
+        return loss.value, g
+
+    def activation(self, name: str, grad: bool = False) -> np.ndarray:
+        i = self.BUFFERS.index(name)
+        lv = self.LEVELS[i]
+        n = int(np.prod([c >> lv for c in self.cells])) * self.CHANNELS[i]
+        out = np.zeros(n, np.float32)
+        _check(lib().aphid_unet_activation(C.c_void_p(self._n), i, int(grad), _p(out)),
+               "aphid_unet_activation")
+        return out.reshape(-1, self.CHANNELS[i])
+
+    def reserve(self, capacity: int):
+        _check(lib().aphid_unet_reserve(C.c_void_p(self._n), capacity), "aphid_unet_reserve")
+
+    def add_sample(self, features, target) -> int:
+        return _check(lib().aphid_unet_add_sample(C.c_void_p(self._n), _p(self._f(features, 3)),
+                                                  _p(self._f(target, 8))), "aphid_unet_add_sample")
+
+    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0):
+        losses = np.zeros(max(steps, 1), np.float32)
+        secs = C.c_double()
+        _check(lib().aphid_unet_train(C.c_void_p(self._n), steps, batch, C.c_float(lr),
+                                      C.c_ulonglong(seed), _p(losses), C.byref(secs)),
+               "aphid_unet_train")
+        return losses[:steps], secs.value
+
+    def time(self, reps: int = 5):
+        f, s = C.c_double(), C.c_double()
+        _check(lib().aphid_unet_time(C.c_void_p(self._n), reps, C.byref(f), C.byref(s)),
+               "aphid_unet_time")
+        return f.value, s.value
+
+
 EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_set_mesh",
            "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
            "aphid_get_diag", "aphid_apply", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
@@ -364,4 +482,9 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy",
            "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",
            "aphid_batch_add_system", "aphid_batch_solve", "aphid_batch_get_solution",
-           "aphid_batch_set_guess"]
+           "aphid_batch_set_guess",
+           "aphid_unet_create", "aphid_unet_destroy", "aphid_unet_param_count",
+           "aphid_unet_forward_flops", "aphid_unet_get_params", "aphid_unet_set_params",
+           "aphid_unet_forward", "aphid_unet_loss_grad", "aphid_unet_activation",
+           "aphid_unet_reserve", "aphid_unet_add_sample", "aphid_unet_train", "aphid_unet_time",
+           "aphid_warm_start"]

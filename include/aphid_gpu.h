@@ -224,6 +224,62 @@ int aphid_batch_get_solution(aphid_batch *batch, int sys, double *x_x,
  * sys (same mesh), e.g. the previous frequency of a sweep. */
 int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
 
+/* ---------------------------------------------------------------------------
+ * Learned warm start.  No counterpart in the reference, which always starts
+ * QMR from x0 = 0 (src/aphid_qmr.f90:230-233).
+ *
+ * A 3-level 3-D U-Net (16/32/64 channels) on a cell grid of nx x ny x nz
+ * cells (multiples of 4), bf16 weights and activations with FP32
+ * accumulation on the matrix cores, trained on the GPU.  Features per cell
+ * (z fastest, FP32, 3 per cell): normalised log10 sigma, skin-depth
+ * normalised log frequency, source mask.  Output per cell: 8 FP32 (re, im
+ * of Ax, Ay, Az, Phi) as a normalised field shape.  Parameters are one flat
+ * FP32 vector: per layer the weights [cout][cin][k][k][k] then the biases.
+ */
+typedef struct aphid_unet aphid_unet;
+
+aphid_unet *aphid_unet_create(int device, int nx, int ny, int nz,
+                              unsigned long long seed);
+void aphid_unet_destroy(aphid_unet *net);
+long long aphid_unet_param_count(const aphid_unet *net);
+/* multiply-add FLOPs (x2) of one forward pass */
+double aphid_unet_forward_flops(const aphid_unet *net);
+int aphid_unet_get_params(aphid_unet *net, float *params);
+int aphid_unet_set_params(aphid_unet *net, const float *params);
+/* out (optional): [nx*ny*nz][8] */
+int aphid_unet_forward(aphid_unet *net, const float *features, float *out);
+/* One forward and backward pass of the mean squared error against
+ * target [nx*ny*nz][8]; grad (optional) receives d loss / d params. */
+int aphid_unet_loss_grad(aphid_unet *net, const float *features,
+                         const float *target, double *loss, float *grad);
+/* A stored tensor of the last pass (index into the network's buffer list;
+ * grad != 0: its gradient), as FP32. */
+int aphid_unet_activation(aphid_unet *net, int which, int grad, float *out);
+/* Training set held in device memory (bf16). */
+int aphid_unet_reserve(aphid_unet *net, int capacity);
+int aphid_unet_add_sample(aphid_unet *net, const float *features,
+                          const float *target);
+/* `steps` Adam steps over random mini-batches of `batch` samples;
+ * losses (optional) gets the mean loss of each step. */
+int aphid_unet_train(aphid_unet *net, int steps, int batch, float lr,
+                     unsigned long long seed, float *losses, double *seconds);
+/* Device time of one forward pass and of one forward+backward pass. */
+int aphid_unet_time(aphid_unet *net, int reps, double *fwd_seconds,
+                    double *step_seconds);
+
+#define APHID_WS_PREV 1u /* include the context's current solution */
+#define APHID_WS_LSQ 2u  /* fit amplitudes by least squares on the residual */
+/* Initial guess for aphid_solve_resident(..., APHID_F_X0) from the network
+ * (net may be NULL) and/or the context's current solution.  One complex
+ * amplitude per candidate (each network component, the previous solution)
+ * from the Galerkin condition u^T (b - K x0) = 0 (default), or minimising
+ * ||b - K x0|| (APHID_WS_LSQ: then never worse than x0 = 0).  Each cell
+ * (i,j,k) of the network owns the unknowns at its lower corner node; the
+ * mesh must have (nx+1, ny+1, nz+1) nodes for nx x ny x nz cells.
+ * r0_rel (optional): ||b - K x0|| / ||b||. */
+int aphid_warm_start(aphid_ctx *ctx, aphid_unet *net, const float *features,
+                     unsigned flags, double *r0_rel);
+
 #ifdef __cplusplus
 }
 #endif

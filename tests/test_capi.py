@@ -10,7 +10,8 @@ from conftest import ROOT
 
 def declared():
     txt = open(os.path.join(ROOT, "include", "aphid_gpu.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(aphid_\w+)\s*\(", txt, re.M)))
+    pat = r"^\s*(?:int|void|double|long long|const char \*|aphid_unet \*)\s*(aphid_\w+)\s*\("
+    return sorted(set(re.findall(pat, txt, re.M)))
 
 
 def test_header_declares_the_binding_surface():

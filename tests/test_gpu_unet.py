@@ -1,0 +1,184 @@
+"""Warm-start network on the GPU vs the CPU reference (oracle/unet_ref.py):
+forward activations layer by layer, the loss gradient, and training."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+pytestmark = pytest.mark.gpu
+
+CELLS = (8, 12, 8)
+
+
+def _net(seed=1):
+    from aphid_git_amd import gpu
+
+    return gpu.UNet(CELLS, seed=seed)
+
+
+def _features(seed):
+    rng = np.random.default_rng(seed)
+    V = int(np.prod(CELLS))
+    f = np.stack([rng.uniform(-1, 1, V), rng.uniform(-1, 1, V), (rng.random(V) < 0.05) * 1.0], -1)
+    return f.astype(np.float32)
+
+
+def _target(seed):
+    rng = np.random.default_rng(seed + 100)
+    X, Y, Z = CELLS
+    x, y, z = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+    t = [np.sin(0.3 * (c + 1) * x + 0.2 * y) * np.cos(0.25 * z + c) for c in range(8)]
+    t = np.stack(t, -1).reshape(-1, 8) + 0.05 * rng.standard_normal((x.size, 8))
+    return t.astype(np.float32)
+
+
+def test_parameter_layout_matches_reference():
+    import unet_ref
+
+    net = _net()
+    assert net.n_params == unet_ref.n_params()
+    p = net.get_params()
+    net.set_params(p * 0.5)
+    np.testing.assert_array_equal(net.get_params(), p * 0.5)
+
+
+def test_forward_matches_cpu_reference_layer_by_layer():
+    import torch
+    import torch.nn.functional as F
+    import unet_ref
+
+    net = _net(3)
+    p = net.get_params()
+    feat = _features(5)
+    out = net.forward(feat)
+    P = unet_ref.split_params(p)
+    # each convolution applied by torch to the GPU's own stored input: the
+    # outputs agree to FP32 accumulation order, i.e. at most one bf16 ulp
+    convs = [("in", "a1", 0, 3), ("a1", "cat1", 1, 3), ("p1", "a2", 2, 3), ("a2", "cat2", 3, 3),
+             ("p2", "a3", 4, 3), ("a3", "b3", 5, 3), ("b3", "r3", 6, 1), ("cat2", "d2a", 7, 3),
+             ("d2a", "d2b", 8, 3), ("d2b", "r2", 9, 1), ("cat1", "d1a", 10, 3),
+             ("d1a", "d1b", 11, 3), ("d1b", "out", 12, 1)]
+    for src, dst, l, k in convs:
+        lv = net.LEVELS[net.BUFFERS.index(src)]
+        dims = [c >> lv for c in CELLS]
+        xin = net.activation(src)
+        if src == "in":
+            xin = xin[:, :3]
+        x = torch.as_tensor(xin.astype(np.float64)).reshape(*dims, -1).permute(3, 0, 1, 2)[None]
+        w, b = P[l]
+        y = F.conv3d(x, unet_ref.rb(w), bias=b, padding=k // 2)
+        if dst != "out":
+            y = F.relu(y)
+        ref = unet_ref.to_voxels(y)
+        got = net.activation(dst)
+        if dst == "cat1":
+            got = got[:, 16:32]
+        elif dst == "cat2":
+            got = got[:, 32:64]
+        if dst == "out":
+            err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30)
+            assert err < 1e-5, (dst, err)
+        else:
+            refb = unet_ref.bf16_round(ref)
+            ulp = np.maximum(np.abs(refb), 1e-30) * 2.0 ** -7
+            assert np.all(np.abs(got - refb) <= ulp + 1e-30), (dst, np.abs(got - refb).max())
+            assert np.mean(got != refb) < 0.01, (dst, np.mean(got != refb))
+    # end to end against the reference's own forward (rounding flips may
+    # propagate, so a norm-wise bound)
+    ref = unet_ref.to_voxels(unet_ref.forward(P, feat, CELLS))
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err < 2e-2, err
+    # pooling and upsampling: the GPU's stored tensors
+    s1 = net.activation("cat1")[:, 16:32]
+    p1 = unet_ref.to_voxels(F.avg_pool3d(torch.as_tensor(s1.astype(np.float64)).reshape(
+        *CELLS, 16).permute(3, 0, 1, 2)[None], 2))
+    got = net.activation("p1")
+    assert np.all(np.abs(got - unet_ref.bf16_round(p1)) <= np.abs(p1) * 2.0 ** -7 + 1e-30)
+    r2 = net.activation("r2")
+    half = [c // 2 for c in CELLS]
+    up = unet_ref.to_voxels(F.interpolate(torch.as_tensor(r2.astype(np.float64)).reshape(
+        *half, 16).permute(3, 0, 1, 2)[None], scale_factor=2, mode="nearest"))
+    np.testing.assert_array_equal(net.activation("cat1")[:, :16], up)
+
+
+def test_gradient_matches_cpu_reference():
+    import unet_ref
+
+    net = _net(4)
+    p = net.get_params()
+    feat, tgt = _features(6), _target(6)
+    loss, g = net.loss_grad(feat, tgt)
+    lref, gref = unet_ref.loss_grad(p, feat, tgt, CELLS)
+    assert abs(loss - lref) <= 2e-3 * lref, (loss, lref)
+    off = 0
+    for l, (k, ci, co) in enumerate(unet_ref.LAYERS):
+        nw = co * ci * k ** 3
+        for name, sl in (("w", slice(off, off + nw)), ("b", slice(off + nw, off + nw + co))):
+            a, r = g[sl], gref[sl]
+            err = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30)
+            assert err < 3e-2, (l, name, err)
+        off += nw + co
+    cos = float(g @ gref) / (np.linalg.norm(g) * np.linalg.norm(gref))
+    assert cos > 0.999, cos
+
+
+def test_training_reduces_the_loss():
+    net = _net(7)
+    net.reserve(4)
+    for q in range(4):
+        net.add_sample(_features(10 + q), _target(10 + q))
+    losses, secs = net.train(60, batch=2, lr=3e-3, seed=1)
+    assert np.all(np.isfinite(losses))
+    assert losses[-10:].mean() < 0.5 * losses[:3].mean(), (losses[:3], losses[-10:])
+
+
+def test_warm_start_maps_the_prediction_and_fits_galerkin_amplitudes():
+    from aphid_git_amd import gpu, meshgen, warmstart
+
+    case = meshgen.random_block_case(CELLS, 11, 30.0)
+    s = gpu.Solver(0)
+    b = s.setup_case(case)
+    feat = warmstart.features(case, b)
+    net = _net(12)
+    pred = net.forward(feat)
+    u = warmstart.cells_to_unknowns(pred, case.shape)
+    s.load_rhs(b)
+    r0 = s.warm_start(net, feat)
+    x0 = s.fetch_solution()
+    # each component of x0 is a complex multiple of the host-mapped prediction
+    alphas = []
+    for c in range(4):
+        a = np.vdot(u[c].ravel(), x0[c].ravel()) / np.vdot(u[c].ravel(), u[c].ravel())
+        np.testing.assert_allclose(x0[c], a * u[c], rtol=1e-9, atol=1e-12 * np.abs(x0[c]).max())
+        alphas.append(a)
+    # Galerkin: u_c^T (b - K x0) = 0 for every component
+    r = [bb - kk for bb, kk in zip(b, s.apply(x0))]
+    bn = np.sqrt(sum(np.sum(np.abs(a) ** 2) for a in b))
+    rn = np.sqrt(sum(np.sum(np.abs(a) ** 2) for a in r))
+    assert abs(rn / bn - r0) <= 1e-9 * max(1.0, r0)
+    for c in range(4):
+        g = np.sum(u[c] * r[c])
+        assert abs(g) <= 1e-8 * np.sqrt(np.sum(np.abs(u[c]) ** 2)) * bn, (c, g)
+    # least squares never exceeds ||b||
+    s.load_rhs(b)
+    assert s.warm_start(net, feat, lsq=True) <= 1.0 + 1e-12
+    # no candidate: x0 = 0; the exact solution as previous guess: alpha = 1
+    s.load_rhs(b)
+    assert s.warm_start() == pytest.approx(1.0)
+    assert all(np.abs(a).max() == 0 for a in s.fetch_solution())
+    res = s.solve_resident(1e-10, 20000)
+    assert res.status == 0
+    xs = s.fetch_solution()
+    r0p = s.warm_start(prev=True)
+    assert r0p <= 2e-10
+    for a, bq in zip(s.fetch_solution(), xs):
+        np.testing.assert_allclose(a, bq, rtol=1e-6, atol=1e-9 * np.abs(bq).max())
+    warm = s.solve_resident(1e-8, 20000, gpu.F_X0)
+    assert warm.status == 0 and warm.iters <= 1  # the loop tests after its first update
+    s.close()
+    net.close()

@@ -1,11 +1,14 @@
 """Pins for the CPU oracle that need no reference run: what the mathematics
 fixes (operator symmetry, uniform-medium closed forms, the dense direct
 solve, and the Krylov characterisation of the QMR iterates)."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 from aphid_git_amd import meshgen
-from conftest import flat4, shapes, split4
+from conftest import ROOT, flat4, shapes, split4
 
 
 def assemble(oracle, S):
@@ -204,3 +207,53 @@ def test_qmr_breakdown_is_a_status_not_a_crash(oracle):
     S = oracle.System(case)
     x, rep, hist = S.solve(tol=1e-8, max_it=50)
     assert rep.status == 2
+
+
+def test_unet_reference_bf16_rounding_matches_torch():
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import unet_ref
+
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal(10000) * 10.0 ** rng.uniform(-30, 30, 10000)).astype(np.float32)
+    ref = torch.as_tensor(a).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(unet_ref.bf16_round(a), ref)
+
+
+def test_unet_reference_gradient_matches_finite_differences():
+    """Pins the reference's autograd path (the straight-through roundings
+    only change values, not the chain rule): with small parameters the
+    rounding-free float64 gradient equals central differences."""
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import unet_ref
+
+    cells = (4, 4, 4)
+    rng = np.random.default_rng(1)
+    flat = (0.2 * rng.standard_normal(unet_ref.n_params())).astype(np.float64)
+    feat = rng.uniform(-1, 1, (64, 3))
+    tgt = rng.standard_normal((64, 8))
+    ident = lambda x: x  # noqa: E731
+    saved = unet_ref.rb, unet_ref.gb
+    unet_ref.rb, unet_ref.gb = ident, ident
+    try:
+        def L(fl):
+            P = unet_ref.split_params(fl)
+            out = unet_ref.to_voxels_t(unet_ref.forward(P, feat, cells))
+            return float(((out - torch.as_tensor(tgt)) ** 2).mean())
+
+        P = unet_ref.split_params(flat, requires_grad=True)
+        out = unet_ref.to_voxels_t(unet_ref.forward(P, feat, cells))
+        ((out - torch.as_tensor(tgt)) ** 2).mean().backward()
+        g = torch.cat([torch.cat([w.grad.reshape(-1), b.grad]) for w, b in P]).numpy()
+        idx = rng.choice(flat.size, 40, replace=False)
+        h = 1e-6
+        for q in idx:
+            e = np.zeros_like(flat)
+            e[q] = h
+            fd = (L(flat + e) - L(flat - e)) / (2 * h)
+            assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(g[q])) + 1e-9, (q, fd, g[q])
+    finally:
+        unet_ref.rb, unet_ref.gb = saved
