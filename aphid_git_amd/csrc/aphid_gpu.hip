@@ -1089,15 +1089,85 @@ int aphid_fetch_solution(aphid_ctx* c, double* x_x, double* x_y, double* x_z, do
   return download4(c, c->V[0], h);
 }
 
+namespace {
+
+// Amplitudes of the candidates in g (Galerkin or least squares), solved on
+// the host by Gaussian elimination with partial pivoting and a tiny
+// relative ridge.  Returns false when the fit is singular or non-finite.
+int ws_fit(aphid_ctx* c, const GramArgs& g, bool lsq, std::vector<c128>& al, bool* ok) {
+  *ok = false;
+  if (lsq)
+    hipLaunchKernelGGL(k_gram<false>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, g, c->part);
+  else
+    hipLaunchKernelGGL(k_gram<true>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, g, c->part);
+  CK(hipGetLastError());
+  std::vector<double> hp((size_t)kRedBlocks * 2 * kWsPairs);
+  CK(hipMemcpyAsync(hp.data(), c->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->s));
+  CK(hipStreamSynchronize(c->s));
+  std::vector<double> tot(2 * kWsPairs, 0.0);
+  for (int b = 0; b < kRedBlocks; ++b)
+    for (int q = 0; q < 2 * kWsPairs; ++q) tot[q] += hp[(size_t)b * 2 * kWsPairs + q];
+  const int nb = g.nb;
+  std::vector<c128> G(nb * nb), h(nb);
+  al.assign(nb, c128{0.0, 0.0});
+  int p = 0;
+  for (int a = 0; a < kWsMax; ++a)
+    for (int b = a; b < kWsMax; ++b, ++p)
+      if (a < nb && b < nb) {
+        G[a * nb + b] = {tot[2 * p], tot[2 * p + 1]};
+        G[b * nb + a] = {tot[2 * p], lsq ? -tot[2 * p + 1] : tot[2 * p + 1]};
+      }
+  for (int a = 0; a < kWsMax; ++a, ++p)
+    if (a < nb) h[a] = {tot[2 * p], tot[2 * p + 1]};
+  auto mag = [](c128 z) { return std::hypot(z.re, z.im); };
+  double dmax = 0.0;
+  for (int a = 0; a < nb; ++a) dmax = std::max(dmax, mag(G[a * nb + a]));
+  if (!(dmax > 0.0) || !std::isfinite(dmax)) return 0;
+  for (int a = 0; a < nb; ++a) G[a * nb + a].re += 1e-13 * dmax;
+  for (int k = 0; k < nb; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < nb; ++i)
+      if (mag(G[i * nb + k]) > mag(G[piv * nb + k])) piv = i;
+    if (!(mag(G[piv * nb + k]) > 0.0)) return 0;
+    if (piv != k) {
+      for (int j = 0; j < nb; ++j) std::swap(G[k * nb + j], G[piv * nb + j]);
+      std::swap(h[k], h[piv]);
+    }
+    for (int i = k + 1; i < nb; ++i) {
+      const c128 f = cdiv(G[i * nb + k], G[k * nb + k]);
+      for (int j = k; j < nb; ++j) {
+        const c128 t = f * G[k * nb + j];
+        G[i * nb + j] = {G[i * nb + j].re - t.re, G[i * nb + j].im - t.im};
+      }
+      const c128 t = f * h[k];
+      h[i] = {h[i].re - t.re, h[i].im - t.im};
+    }
+  }
+  for (int i = nb - 1; i >= 0; --i) {
+    c128 sacc = h[i];
+    for (int j = i + 1; j < nb; ++j) {
+      const c128 t = G[i * nb + j] * al[j];
+      sacc = {sacc.re - t.re, sacc.im - t.im};
+    }
+    al[i] = cdiv(sacc, G[i * nb + i]);
+    if (!finite_c(al[i])) return 0;
+  }
+  *ok = true;
+  return 0;
+}
+
+}  // namespace
+
 // Initial guess for aphid_solve_resident(..., APHID_F_X0).  The candidate
 // vectors are the network's four components (each a block of its mapped
 // prediction) and/or the current solution; their amplitudes come from
 //  - the Galerkin condition u_a^T (b - K x0) = 0 for every candidate u_a
 //    (default: exact for an exact guess, and insensitive to small-scale
-//    error that the residual norm would amplify), or
-//  - least squares on ||b - K x0|| (APHID_WS_LSQ: never above ||b||).
-// The small linear system is solved on the host (Gaussian elimination with
-// partial pivoting, tiny relative ridge).  A non-finite fit falls back to 0.
+//    error that the residual norm amplifies), kept only when the true
+//    residual ||b - K x0|| does not exceed ||b||, otherwise
+//  - least squares on ||b - K x0|| (APHID_WS_LSQ forces it), which never
+//    exceeds ||b|| (x0 = 0 is among its candidates).
+// A singular or non-finite fit gives x0 = 0.
 int aphid_warm_start(aphid_ctx* c, aphid_unet* net, const float* feat, unsigned flags,
                      double* r0_rel) {
   if (!c) return fail(APHID_E_ARG, "NULL context");
@@ -1105,7 +1175,6 @@ int aphid_warm_start(aphid_ctx* c, aphid_unet* net, const float* feat, unsigned 
   if (!c->V[1]) return fail(APHID_E_STATE, "aphid_load_rhs first");
   if (net && !feat) return fail(APHID_E_ARG, "aphid_warm_start: features required with a network");
   const bool use_prev = flags & APHID_WS_PREV;
-  const bool lsq = flags & APHID_WS_LSQ;
   CK(hipSetDevice(c->dev));
   int rc = autotune(c, 0);
   if (rc) return rc;
@@ -1135,96 +1204,44 @@ int aphid_warm_start(aphid_ctx* c, aphid_unet* net, const float* feat, unsigned 
       g.w[g.nb++] = c->V[2 + q];
     }
   }
+  // the network's images are in V[2..5]; the previous solution is kept in
+  // V[6] and its image in V[8] (which true_residual reuses, so it is
+  // recomputed before a second fit)
   if (use_prev) {
-    CK(launch_apply<double>(c->tile[0], c->op64, c->V[0], c->V[6], ep, c->s));
-    g.u[g.nb] = c->V[0];
+    CK(hipMemcpyAsync(c->V[6], c->V[0], sizeof(c128) * n4, hipMemcpyDeviceToDevice, c->s));
+    g.u[g.nb] = c->V[6];
     g.ublk[g.nb] = -1;
-    g.w[g.nb++] = c->V[6];
-  }
-  WsCoef cf{};
-  cf.use_net = net != nullptr;
-  cf.use_prev = use_prev;
-  bool fitted = g.nb == 0;
-  if (g.nb > 0) {
-    if (lsq)
-      hipLaunchKernelGGL(k_gram<false>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, g, c->part);
-    else
-      hipLaunchKernelGGL(k_gram<true>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, g, c->part);
-    CK(hipGetLastError());
-    std::vector<double> hp((size_t)kRedBlocks * 2 * kWsPairs);
-    CK(hipMemcpyAsync(hp.data(), c->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->s));
-    CK(hipStreamSynchronize(c->s));
-    std::vector<double> tot(2 * kWsPairs, 0.0);
-    for (int b = 0; b < kRedBlocks; ++b)
-      for (int q = 0; q < 2 * kWsPairs; ++q) tot[q] += hp[(size_t)b * 2 * kWsPairs + q];
-    // G (nb x nb) and h from the packed sums: Hermitian (lsq) or symmetric
-    const int nb = g.nb;
-    std::vector<c128> G(nb * nb), h(nb), al(nb);
-    int p = 0;
-    for (int a = 0; a < kWsMax; ++a)
-      for (int b = a; b < kWsMax; ++b, ++p)
-        if (a < nb && b < nb) {
-          G[a * nb + b] = {tot[2 * p], tot[2 * p + 1]};
-          G[b * nb + a] = {tot[2 * p], lsq ? -tot[2 * p + 1] : tot[2 * p + 1]};
-        }
-    for (int a = 0; a < kWsMax; ++a, ++p)
-      if (a < nb) h[a] = {tot[2 * p], tot[2 * p + 1]};
-    double dmax = 0.0;
-    for (int a = 0; a < nb; ++a) dmax = std::max(dmax, std::hypot(G[a * nb + a].re, G[a * nb + a].im));
-    if (dmax > 0.0 && std::isfinite(dmax)) {
-      for (int a = 0; a < nb; ++a) G[a * nb + a].re += 1e-13 * dmax;
-      fitted = true;
-      for (int k = 0; k < nb && fitted; ++k) {  // elimination with partial pivoting
-        int piv = k;
-        for (int i = k + 1; i < nb; ++i)
-          if (std::hypot(G[i * nb + k].re, G[i * nb + k].im) >
-              std::hypot(G[piv * nb + k].re, G[piv * nb + k].im))
-            piv = i;
-        if (!(std::hypot(G[piv * nb + k].re, G[piv * nb + k].im) > 0.0)) {
-          fitted = false;
-          break;
-        }
-        if (piv != k) {
-          for (int j = 0; j < nb; ++j) std::swap(G[k * nb + j], G[piv * nb + j]);
-          std::swap(h[k], h[piv]);
-        }
-        for (int i = k + 1; i < nb; ++i) {
-          const c128 f = cdiv(G[i * nb + k], G[k * nb + k]);
-          for (int j = k; j < nb; ++j) {
-            const c128 t = f * G[k * nb + j];
-            G[i * nb + j] = {G[i * nb + j].re - t.re, G[i * nb + j].im - t.im};
-          }
-          const c128 t = f * h[k];
-          h[i] = {h[i].re - t.re, h[i].im - t.im};
-        }
-      }
-      for (int i = nb - 1; i >= 0 && fitted; --i) {
-        c128 sacc = h[i];
-        for (int j = i + 1; j < nb; ++j) {
-          const c128 t = G[i * nb + j] * al[j];
-          sacc = {sacc.re - t.re, sacc.im - t.im};
-        }
-        al[i] = cdiv(sacc, G[i * nb + i]);
-        fitted = finite_c(al[i]);
-      }
-      if (fitted) {
-        int q = 0;
-        if (net)
-          for (int k = 0; k < 4; ++k) cf.a[k] = al[q++];
-        if (use_prev) cf.prev = al[q++];
-      }
-    }
-  }
-  if (fitted) {
-    hipLaunchKernelGGL(k_ws_combine, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, nn,
-                       (const c128*)c->V[7], c->V[0], cf);
-    CK(hipGetLastError());
-  } else {
-    CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
+    g.w[g.nb++] = c->V[8];
   }
   double rel = 1.0, rn = 0.0, bn = 0.0;
-  rc = true_residual(c, &rel, &rn, &bn);
-  if (rc) return rc;
+  bool lsq = (flags & APHID_WS_LSQ) != 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    WsCoef cf{};
+    bool ok = g.nb == 0;
+    std::vector<c128> al;
+    if (use_prev) CK(launch_apply<double>(c->tile[0], c->op64, c->V[6], c->V[8], ep, c->s));
+    if (g.nb > 0) {
+      rc = ws_fit(c, g, lsq, al, &ok);
+      if (rc) return rc;
+    }
+    if (ok && g.nb > 0) {
+      int q = 0;
+      cf.use_net = net != nullptr;
+      cf.use_prev = use_prev;
+      if (net)
+        for (int k = 0; k < 4; ++k) cf.a[k] = al[q++];
+      if (use_prev) cf.prev = al[q++];
+      hipLaunchKernelGGL(k_ws_combine, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, nn,
+                         (const c128*)c->V[7], (const c128*)c->V[6], c->V[0], cf);
+      CK(hipGetLastError());
+    } else {
+      CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
+    }
+    rc = true_residual(c, &rel, &rn, &bn);
+    if (rc) return rc;
+    if (std::isfinite(rel) && (rel <= 1.0 || lsq)) break;
+    lsq = true;  // the judge: fall back to the least-squares amplitudes
+  }
   if (!std::isfinite(rel)) {
     CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
     CK(hipStreamSynchronize(c->s));

@@ -255,8 +255,8 @@ int conv_bwd(aphid_unet* n, int l, const bf16_t* in_override, bool need_dx) {
                      sp.mode == 0 ? (const bf16_t*)n->buf[sp.out] : (const bf16_t*)nullptr,
                      kBufs[sp.out].ch, sp.out_off, n->gy, n->bpart);
   UCK(hipGetLastError());
-  hipLaunchKernelGGL(kc_bias_reduce, dim3(1), dim3(64), 0, n->s, (const float*)n->bpart, nbb, C,
-                     n->g + L.boff);
+  hipLaunchKernelGGL(kc_bias_reduce, dim3(C), dim3(kBlock), 0, n->s, (const float*)n->bpart, nbb,
+                     C, n->g + L.boff);
   UCK(hipGetLastError());
   DwArgs d;
   d.X = X; d.Y = Y; d.Z = Z; d.V = V;
@@ -269,7 +269,7 @@ int conv_bwd(aphid_unet* n, int l, const bf16_t* in_override, bool need_dx) {
   const dim3 grid((unsigned)((L.kpad + 255) / 256), (unsigned)L.nsplit);
   UCK(launch_dw(sp.ks, sp.cinp, sp.cout, grid, d, n->s));
   const long long nw = (long long)sp.cout * sp.cin * L.nt;
-  hipLaunchKernelGGL(kc_dw_reduce, dim3(grid_for(nw)), dim3(kBlock), 0, n->s,
+  hipLaunchKernelGGL(kc_dw_reduce, dim3((unsigned)((nw + 63) / 64)), dim3(kBlock), 0, n->s,
                      (const float*)n->dwpart, L.nsplit, L.coutp, L.kpad, sp.cout, sp.cin, sp.cinp,
                      L.nt, n->g + L.woff);
   UCK(hipGetLastError());
@@ -327,8 +327,8 @@ int loss(aphid_unet* n, const bf16_t* tgt, int batch) {
   hipLaunchKernelGGL(kc_loss, dim3(nb), dim3(kBlock), 0, n->s, ne, (const float*)n->buf[B_OUT], tgt,
                      (float)scale, n->dbuf[B_OUT], n->lpart);
   UCK(hipGetLastError());
-  hipLaunchKernelGGL(kc_loss_reduce, dim3(1), dim3(64), 0, n->s, (const double*)n->lpart, nb, scale,
-                     n->lacc);
+  hipLaunchKernelGGL(kc_loss_reduce, dim3(1), dim3(kBlock), 0, n->s, (const double*)n->lpart, nb,
+                     scale, n->lacc);
   UCK(hipGetLastError());
   return 0;
 }
@@ -406,7 +406,7 @@ int create(aphid_unet* n, int dev, int nx, int ny, int nz, unsigned long long se
     off += L.sp.cout;
     const int V = n->V[L.sp.level];
     const int ntile = (L.kpad + 255) / 256;
-    const int want = std::max(1, 2048 / ntile);
+    const int want = std::max(1, 512 / ntile);  // ~2 blocks per CU
     const int maxs = std::max(1, (V + 31) / 32);
     int ns = std::min(want, maxs);
     L.vper = cpad((V + ns - 1) / ns, 32);

@@ -254,16 +254,31 @@ __global__ __launch_bounds__(256) void kc_dw(DwArgs a) {
 }
 
 // grad[(co*cin_real + ci)*NT + tap] += sum_s part[s][co][tap*cinp + ci]
-__global__ void kc_dw_reduce(const float* __restrict__ part, int nsplit, int coutp, int kpad,
-                             int cout, int cin_real, int cinp, int nt, float* __restrict__ grad) {
+// A block of 256 threads owns 64 consecutive outputs; the 4 threads of an
+// output sum the splits congruent to their index mod 4, then combine in a
+// fixed order (deterministic).
+__global__ __launch_bounds__(256) void kc_dw_reduce(const float* __restrict__ part, int nsplit,
+                                                    int coutp, int kpad, int cout, int cin_real,
+                                                    int cinp, int nt, float* __restrict__ grad) {
+  __shared__ float red[4][64];
   const int n = cout * cin_real * nt;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+  const int o = threadIdx.x % 64, g = threadIdx.x / 64;
+  const int q = blockIdx.x * 64 + o;
+  float acc = 0.f;
+  size_t e = 0;
+  if (q < n) {
     const int tap = q % nt, ci = (q / nt) % cin_real, co = q / (nt * cin_real);
-    const size_t e = (size_t)co * kpad + tap * cinp + ci;
-    float acc = 0.f;
-    for (int s = 0; s < nsplit; ++s) acc += part[(size_t)s * coutp * kpad + e];
-    grad[q] += acc;
+    e = (size_t)co * kpad + tap * cinp + ci;
+    const size_t stride = (size_t)coutp * kpad;
+    int s = g;
+    for (; s + 12 < nsplit; s += 16)
+      acc += (part[s * stride + e] + part[(s + 4) * stride + e]) +
+             (part[(s + 8) * stride + e] + part[(s + 12) * stride + e]);
+    for (; s < nsplit; s += 4) acc += part[s * stride + e];
   }
+  red[g][o] = acc;
+  __syncthreads();
+  if (g == 0 && q < n) grad[q] += (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,13 +311,21 @@ __global__ __launch_bounds__(256) void kc_relu_bwd(int V, int C, const float* __
   }
 }
 
-// db[c] += sum_b part[b][c]  (fixed order)
-__global__ void kc_bias_reduce(const float* __restrict__ part, int nb, int C, float* __restrict__ db) {
-  const int c = threadIdx.x;
-  if (c >= C) return;
+// db[c] += sum_b part[b][c]: one block of 256 threads per channel, then a
+// fixed-order tree (deterministic)
+__global__ __launch_bounds__(256) void kc_bias_reduce(const float* __restrict__ part, int nb, int C,
+                                                      float* __restrict__ db) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
   float t = 0.f;
-  for (int b = 0; b < nb; ++b) t += part[(size_t)b * C + c];
-  db[c] += t;
+  for (int b = threadIdx.x; b < nb; b += 256) t += part[(size_t)b * C + c];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] += red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -396,12 +419,19 @@ __global__ __launch_bounds__(256) void kc_loss(long long n, const float* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void kc_loss_reduce(const double* __restrict__ part, int nb, double scale,
-                               double* __restrict__ acc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// *acc += scale * sum_b part[b]: one block of 256 threads, fixed-order tree
+__global__ __launch_bounds__(256) void kc_loss_reduce(const double* __restrict__ part, int nb,
+                                                      double scale, double* __restrict__ acc) {
+  __shared__ double red[256];
   double t = 0.0;
-  for (int b = 0; b < nb; ++b) t += part[b];
-  *acc += t * scale;
+  for (int b = threadIdx.x; b < nb; b += 256) t += part[b];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += red[0] * scale;
 }
 
 // ---------------------------------------------------------------------------

@@ -95,15 +95,15 @@ struct WsCoef {
   int use_net, use_prev;
 };
 
-// x = sum_c a_c u_c + prev * x   (u_c: block c of u)
-__global__ void k_ws_combine(long long nn, const c128* __restrict__ u, c128* __restrict__ x,
-                             WsCoef cf) {
+// x = sum_c a_c u_c + prev * xp   (u_c: block c of u)
+__global__ void k_ws_combine(long long nn, const c128* __restrict__ u,
+                             const c128* __restrict__ xp, c128* __restrict__ x, WsCoef cf) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < 4 * nn;
        q += (long long)gridDim.x * blockDim.x) {
     const int c = (int)(q / nn);
     c128 v = {0.0, 0.0};
     if (cf.use_net) v = cf.a[c] * u[q];
-    if (cf.use_prev) v += cf.prev * x[q];
+    if (cf.use_prev) v += cf.prev * xp[q];
     x[q] = v;
   }
 }

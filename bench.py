@@ -15,6 +15,16 @@ iteration with HIP events on the library's stream (roofline against the
 8 TB/s HBM peak), and a CPU baseline: the reference's own Fortran/OpenMP
 QMR (compiled from its sources into oracle/_ref) timed on this host on a
 bounded sample of the same system.
+
+Further parts (--parts, default all), reported as extra JSON fields:
+  batch  the 33^3-node model family at 64 frequencies (1e-2..1e5 Hz) x 3
+         dipoles = 192 systems: one batched solve vs one solve at a time;
+  unet   the warm-start network on a 96x96x80 cell grid: forward and
+         forward+backward time, TFLOP/s against the 2.5 PFLOP/s dense bf16
+         matrix-core peak;
+  warm   a small learned warm start end to end (16^3 cells): solver-made
+         training set, training on the GPU, iterations vs x0 = 0 on
+         held-out models.
 """
 from __future__ import annotations
 
@@ -33,8 +43,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
+PARTS = ("solve", "batch", "unet", "warm", "cpu")
 
 
 def log(msg):
@@ -92,6 +104,95 @@ def cpu_baseline(case, gpu_iters: float, sample_its: int = 30):
                       f"{gpu_iters:.0f} iterations", "s_per_iteration": s_it}
 
 
+def bench_batch():
+    """Config-1 family x 64 frequencies x 3 dipoles: batched vs sequential."""
+    from aphid_git_amd import gpu, meshgen
+
+    base = meshgen.config1()
+    freqs = np.logspace(-2, 5, 64)
+    rng = np.random.default_rng(3)
+    nx, ny, nz = base.shape
+    pos = [(int(rng.integers(4, nx - 4)), int(rng.integers(4, ny - 4))) for _ in range(3)]
+    srcs = [(1, pos[0][0], pos[0][1], 9), (2, pos[1][0], pos[1][1], 9), (3, pos[2][0], pos[2][1], 9)]
+    single = gpu.Solver(0)
+    single.set_mesh(base.xn, base.yn, base.zn)
+    bs = [single.build_rhs(5, [q])[0] for q in srcs]
+    # one system at a time (warm the tuner and graphs on the first)
+    single.set_cells("R", 2 * np.pi * freqs[0], base.cells)
+    single.load_rhs(bs[0])
+    single.solve_resident(TOL, 20000)
+    t0 = time.perf_counter()
+    conv_seq, its_seq = 0, []
+    for f in freqs:
+        single.set_cells("R", 2 * np.pi * f, base.cells)
+        for q in range(3):
+            single.load_rhs(bs[q])
+            r = single.solve_resident(TOL, 20000)
+            conv_seq += int(r.status == 0 and r.true_rel_res <= TOL)
+            its_seq.append(r.iters)
+    t_seq = time.perf_counter() - t0
+    single.close()
+    B = gpu.BatchSolver(0)
+    t0 = time.perf_counter()
+    ops = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells) for f in freqs]
+    for o in ops:
+        for q in range(3):
+            B.add_system(o, bs[q])
+    t_setup = time.perf_counter() - t0
+    B.solve(TOL, 20000)  # warm: graph capture and tiles
+    t0 = time.perf_counter()
+    reps, secs = B.solve(TOL, 20000)
+    t_b = time.perf_counter() - t0
+    B.close()
+    conv_b = sum(1 for r in reps if r["status"] == 0 and r["true_rel_res"] <= TOL)
+    return {"workload": "33x33x33-node model family (config 1), 64 frequencies 1e-2..1e5 Hz x "
+                        "3 unit dipoles (x, y, z) at seeded surface nodes = 192 systems, rel-res 1e-8",
+            "systems": len(reps), "converged_batched": conv_b, "converged_sequential": conv_seq,
+            "seconds_batched": t_b, "seconds_sequential": t_seq, "seconds_batch_setup": t_setup,
+            "solves_per_s_batched": conv_b / t_b, "solves_per_s_sequential": conv_seq / t_seq,
+            "speedup": (conv_b / t_b) / (conv_seq / t_seq),
+            "iters_mean": float(np.mean([r["iters"] for r in reps])),
+            "iters_max": int(max(r["iters"] for r in reps)),
+            "iters_mean_sequential": float(np.mean(its_seq))}
+
+
+def bench_unet(reps: int = 5):
+    from aphid_git_amd import gpu
+
+    net = gpu.UNet((96, 96, 80), seed=0)
+    f, st = net.time(reps)
+    F = net.forward_flops
+    net.close()
+    return {"workload": "3-level 3-D U-Net 16/32/64 channels on a 96x96x80 cell grid, bf16 "
+                        "weights/activations, FP32 accumulation, batch 1",
+            "params": net.n_params, "forward_gflop": F / 1e9, "forward_ms": 1e3 * f,
+            "forward_tflops": F / f / 1e12, "train_step_ms": 1e3 * st,
+            "train_step_tflops": 3 * F / st / 1e12,
+            "roofline": {"bound": "mfma", "achieved": F / f / 1e12, "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": F / f / 1e12 / BF16_PEAK_TFLOPS}}
+
+
+def bench_warm(ncell=(16, 16, 16), ntrain=64, steps=400, neval=8):
+    from aphid_git_amd import gpu, warmstart
+
+    net = gpu.UNet(ncell, seed=0)
+    net.reserve(ntrain)
+    data = warmstart.build_dataset(net, ncell, range(1000, 1000 + ntrain))
+    losses, t_train = net.train(steps, batch=4, lr=2e-3, seed=1)
+    ev = warmstart.evaluate(net, ncell, range(5000, 5000 + neval))
+    net.close()
+    iz = sum(r["iters_zero"] for r in ev)
+    iw = sum(r["iters_warm"] for r in ev)
+    return {"workload": f"random-block models on {ncell[0]}x{ncell[1]}x{ncell[2]} cells, "
+                        f"frequencies log-uniform 1e-2..1e5 Hz, +X surface dipole",
+            "train_samples": ntrain, "data_seconds": data["seconds"], "train_steps": steps,
+            "train_seconds": t_train, "loss_first": float(losses[:5].mean()),
+            "loss_last": float(losses[-20:].mean()), "eval_systems": neval,
+            "iters_zero_mean": iz / neval, "iters_warm_mean": iw / neval, "iters_ratio": iw / iz,
+            "all_warm_converged": all(r["status_warm"] == 0 and r["true_warm"] <= TOL for r in ev),
+            "r0_median": float(np.median([r["r0"] for r in ev]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +203,12 @@ def main():
     ap.add_argument("--max-it", type=int, default=MAX_IT,
                     help="iteration cap per solve (profiling runs use a small cap; "
                          "a capped solve does not count as converged)")
+    ap.add_argument("--parts", default=",".join(PARTS),
+                    help="comma-separated subset of " + ",".join(PARTS))
     args = ap.parse_args()
+    parts = set(args.parts.split(","))
+    if args.no_cpu:
+        parts.discard("cpu")
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); systems of this solve "
               f"shard one per GPU with no communication, so N GPUs scale the single-GPU rate.",
@@ -114,75 +220,14 @@ def main():
     case = meshgen.config2()
     M = case.n_unknowns
     nx, ny, nz = case.shape
-    log(f"[bench] config2 {case.shape} nodes, M={M}")
-    s = gpu.Solver(0)
-    b = s.setup_case(case)
-    s.load_rhs(b)
-    for w in range(args.warmup):
-        r = s.solve_resident(TOL, args.max_it)
-        log(f"[bench] warmup {w}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
-            f"{r.seconds_qmr:.3f}s")
-
-    results = []
-    t0 = time.perf_counter()  # each solve returns after its stream has drained
-    for k in range(args.steps):
-        r = s.solve_resident(TOL, args.max_it)
-        results.append(r)
-        log(f"[bench] step {k}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
-            f"{r.seconds_qmr:.3f}s")
-    elapsed = time.perf_counter() - t0
-    converged = sum(1 for r in results if r.status == 0 and r.true_rel_res <= TOL)
-    iters = float(np.mean([r.iters for r in results]))
-
-    # kernel timing (HIP events on the library's stream)
-    t_apply = s.time_apply(0, 20)
-    kt = s.time_iteration(0, 10)
-    tile = s.get_tile(0)
-    s_bytes = 16
-    B_mv = 3 * M * s_bytes
-    per_kernel = {
-        "p_update": (4 * M * s_bytes, kt[0]),
-        "stencil_eps": (B_mv, kt[1]),
-        "vt_norm": (2 * M * s_bytes, kt[2]),
-        "fused_update": (13 * M * s_bytes, kt[3]),
-    }
-    kernels = {k: {"algorithmic_bytes": int(bb), "seconds": t, "GBps": bb / t / 1e9,
-                   "frac": bb / t / 1e9 / HBM_PEAK_GBS} for k, (bb, t) in per_kernel.items()}
-    dom = max(per_kernel, key=lambda k: per_kernel[k][1])
-    copy_bytes = float(1 << 30)
-    t_copy = s.time_copy(copy_bytes, 10)
-    copy_probe = {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "seconds": t_copy,
-                  "GBps": 2 * copy_bytes / t_copy / 1e9,
-                  "frac_of_peak": 2 * copy_bytes / t_copy / 1e9 / HBM_PEAK_GBS}
-    it_bytes = 22 * M * s_bytes
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_config2.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("stencil_bytes_per_launch")
-    roof = {"bound": "hbm", "kernel": "stencil (k_apply, matvec)",
-            "achieved": B_mv / t_apply / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": B_mv / t_apply / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": B_mv, "seconds_per_launch": t_apply,
-            "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]}}
-
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU baseline (reference, 4 threads) ...")
-        try:
-            cpu = cpu_baseline(case, iters, args.cpu_its)
-        except Exception as e:  # reported, never fatal
-            cpu = {"error": str(e)[:300]}
-        log(f"[bench] cpu: {cpu}")
-
     out = {
         "metric": "converged solves/s at rel-res 1e-8 (128x128x96 grid); matvec HBM GB/s vs peak",
-        "value": converged / elapsed,
+        "value": None,
         "unit": "converged solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -190,21 +235,95 @@ def main():
         "data": "synthetic (seeded block model, config2)",
         "config": {"workload": "config2: 128x128x96-cell stretched mesh, 64x64x48 core at 25 m, "
                                "20 random blocks, +X dipole, 1 Hz, Jacobi-QMR to rel-res 1e-8",
-                   "nodes": [nx, ny, nz], "unknowns": M, "tol": TOL,
-                   "iterations_per_solve": iters,
-                   "converged_steps": converged,
-                   "true_rel_res_max": max(r.true_rel_res for r in results)},
-        "roofline": roof,
-        "roofline_dominant_kernel": {"kernel": dom, **kernels[dom], "peak": HBM_PEAK_GBS},
-        "kernels": kernels,
-        "iteration": {"seconds": kt[4], "algorithmic_bytes": it_bytes,
-                      "GBps": it_bytes / kt[4] / 1e9,
-                      "frac": it_bytes / kt[4] / 1e9 / HBM_PEAK_GBS},
-        "hbm_copy_probe": copy_probe,
-        "cpu_baseline": cpu,
+                   "nodes": [nx, ny, nz], "unknowns": M, "tol": TOL},
     }
+    iters = None
+    if "solve" in parts:
+        log(f"[bench] config2 {case.shape} nodes, M={M}")
+        s = gpu.Solver(0)
+        b = s.setup_case(case)
+        s.load_rhs(b)
+        for w in range(args.warmup):
+            r = s.solve_resident(TOL, args.max_it)
+            log(f"[bench] warmup {w}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
+                f"{r.seconds_qmr:.3f}s")
+        results = []
+        t0 = time.perf_counter()  # each solve returns after its stream has drained
+        for k in range(args.steps):
+            r = s.solve_resident(TOL, args.max_it)
+            results.append(r)
+            log(f"[bench] step {k}: status {r.status} iters {r.iters} true {r.true_rel_res:.2e} "
+                f"{r.seconds_qmr:.3f}s")
+        elapsed = time.perf_counter() - t0
+        converged = sum(1 for r in results if r.status == 0 and r.true_rel_res <= TOL)
+        iters = float(np.mean([r.iters for r in results]))
+
+        # kernel timing (HIP events on the library's stream)
+        t_apply = s.time_apply(0, 20)
+        kt = s.time_iteration(0, 10)
+        tile = s.get_tile(0)
+        s_bytes = 16
+        B_mv = 3 * M * s_bytes
+        per_kernel = {
+            "p_update": (4 * M * s_bytes, kt[0]),
+            "stencil_eps": (B_mv, kt[1]),
+            "vt_norm": (2 * M * s_bytes, kt[2]),
+            "fused_update": (13 * M * s_bytes, kt[3]),
+        }
+        kernels = {k: {"algorithmic_bytes": int(bb), "seconds": t, "GBps": bb / t / 1e9,
+                       "frac": bb / t / 1e9 / HBM_PEAK_GBS} for k, (bb, t) in per_kernel.items()}
+        dom = max(per_kernel, key=lambda k: per_kernel[k][1])
+        copy_bytes = float(1 << 30)
+        t_copy = s.time_copy(copy_bytes, 10)
+        copy_probe = {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "seconds": t_copy,
+                      "GBps": 2 * copy_bytes / t_copy / 1e9,
+                      "frac_of_peak": 2 * copy_bytes / t_copy / 1e9 / HBM_PEAK_GBS}
+        it_bytes = 22 * M * s_bytes
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic_config2.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get("stencil_bytes_per_launch")
+        s.close()
+        out.update({
+            "value": converged / elapsed,
+            "ms_per_step": 1e3 * elapsed / args.steps,
+            "roofline": {"bound": "hbm", "kernel": "stencil (k_apply, matvec)",
+                         "achieved": B_mv / t_apply / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": B_mv / t_apply / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                         "algorithmic_bytes_per_launch": B_mv, "seconds_per_launch": t_apply,
+                         "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]}},
+            "roofline_dominant_kernel": {"kernel": dom, **kernels[dom], "peak": HBM_PEAK_GBS},
+            "kernels": kernels,
+            "iteration": {"seconds": kt[4], "algorithmic_bytes": it_bytes,
+                          "GBps": it_bytes / kt[4] / 1e9,
+                          "frac": it_bytes / kt[4] / 1e9 / HBM_PEAK_GBS},
+            "hbm_copy_probe": copy_probe,
+        })
+        out["config"].update({"iterations_per_solve": iters, "converged_steps": converged,
+                              "true_rel_res_max": max(r.true_rel_res for r in results)})
+    if "batch" in parts:
+        log("[bench] batch: 192 systems of the config-1 family ...")
+        out["batch_config3a"] = bench_batch()
+        log(f"[bench] batch: {out['batch_config3a']}")
+    if "unet" in parts:
+        log("[bench] warm-start network at 96x96x80 cells ...")
+        out["unet"] = bench_unet()
+        log(f"[bench] unet: {out['unet']}")
+    if "warm" in parts:
+        log("[bench] learned warm start, 16^3 cells ...")
+        out["warm_start"] = bench_warm()
+        log(f"[bench] warm: {out['warm_start']}")
+    cpu = None
+    if "cpu" in parts and iters is not None:
+        log("[bench] CPU baseline (reference, 4 threads) ...")
+        try:
+            cpu = cpu_baseline(case, iters, args.cpu_its)
+        except Exception as e:  # reported, never fatal
+            cpu = {"error": str(e)[:300]}
+        log(f"[bench] cpu: {cpu}")
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
-    s.close()
 
 
 if __name__ == "__main__":

@@ -272,8 +272,9 @@ int aphid_unet_time(aphid_unet *net, int reps, double *fwd_seconds,
 /* Initial guess for aphid_solve_resident(..., APHID_F_X0) from the network
  * (net may be NULL) and/or the context's current solution.  One complex
  * amplitude per candidate (each network component, the previous solution)
- * from the Galerkin condition u^T (b - K x0) = 0 (default), or minimising
- * ||b - K x0|| (APHID_WS_LSQ: then never worse than x0 = 0).  Each cell
+ * from the Galerkin condition u^T (b - K x0) = 0, kept only if
+ * ||b - K x0|| <= ||b||; otherwise (or with APHID_WS_LSQ) minimising
+ * ||b - K x0||, which is never worse than x0 = 0.  Each cell
  * (i,j,k) of the network owns the unknowns at its lower corner node; the
  * mesh must have (nx+1, ny+1, nz+1) nodes for nx x ny x nz cells.
  * r0_rel (optional): ||b - K x0|| / ||b||. */
