@@ -47,6 +47,7 @@ BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
 PARTS = ("solve", "batch", "unet", "warm", "cpu")
+EXTRA_PARTS = ("batch3b",)  # not in the default set (about a minute of GPU time)
 
 
 def log(msg):
@@ -156,6 +157,57 @@ def bench_batch():
             "iters_mean_sequential": float(np.mean(its_seq))}
 
 
+def bench_batch3b(max_it: int = 10000):
+    """48 random-block models, 16 each on 64^3, 96^3 and 128^3 cells, each at
+    one log-uniform frequency in 1e-1..1e4 Hz: one packed batched solve vs
+    one solve at a time."""
+    from aphid_git_amd import gpu, meshgen
+
+    rng = np.random.default_rng(35)
+    cases = []
+    for n in (64, 96, 128):
+        for q in range(16):
+            f = float(10 ** rng.uniform(-1, 4))
+            cases.append(meshgen.random_block_case((n, n, n), 3500 + 100 * n + q, f))
+    m_tot = sum(c.n_unknowns for c in cases)
+    single = gpu.Solver(0)
+    rhs = []
+    for c in cases:
+        rhs.append(single.setup_case(c))
+    single.load_rhs(rhs[-1])  # the solver holds the last case's mesh
+    single.solve_resident(TOL, 50)  # warm the tuner
+    seq = []
+    t_seq = 0.0  # solve time only: each case's setup is excluded, as for the batch
+    for q, (c, b) in enumerate(zip(cases, rhs)):
+        single.setup_case(c)
+        single.load_rhs(b)
+        r = single.solve_resident(TOL, max_it)
+        t_seq += r.seconds_total
+        seq.append(r)
+        log(f"[bench] batch3b sequential {q}: {c.shape} f {c.freq:.3g} iters {r.iters} "
+            f"status {r.status} {r.seconds_total:.2f}s")
+    single.close()
+    B = gpu.BatchSolver(0)
+    ops = [B.add_case_operator(c) for c in cases]  # every operator before the systems
+    for o, b in zip(ops, rhs):
+        B.add_system(o, b)
+    B.solve(TOL, 20)  # warm: graph capture and tiles
+    t0 = time.perf_counter()
+    reps, secs = B.solve(TOL, max_it)
+    t_b = time.perf_counter() - t0
+    B.close()
+    conv_b = sum(1 for r in reps if r["status"] == 0 and r["true_rel_res"] <= TOL)
+    conv_s = sum(1 for r in seq if r.status == 0 and r.true_rel_res <= TOL)
+    return {"workload": "16 each of 64^3, 96^3, 128^3-cell random-block models (50 m cells), "
+                        "one log-uniform frequency each in 1e-1..1e4 Hz, +X surface dipole",
+            "systems": len(cases), "unknowns_total": int(m_tot),
+            "converged_batched": conv_b, "converged_sequential": conv_s,
+            "seconds_batched": t_b, "seconds_sequential": t_seq,
+            "speedup": (conv_b / t_b) / max(conv_s / t_seq, 1e-30),
+            "iters_by_size": {str(n): [int(r["iters"]) for r in reps[16 * q:16 * q + 16]]
+                              for q, n in enumerate((64, 96, 128))}}
+
+
 def bench_unet(reps: int = 5):
     from aphid_git_amd import gpu
 
@@ -204,7 +256,7 @@ def main():
                     help="iteration cap per solve (profiling runs use a small cap; "
                          "a capped solve does not count as converged)")
     ap.add_argument("--parts", default=",".join(PARTS),
-                    help="comma-separated subset of " + ",".join(PARTS))
+                    help="comma-separated subset of " + ",".join(PARTS + EXTRA_PARTS))
     args = ap.parse_args()
     parts = set(args.parts.split(","))
     if args.no_cpu:
@@ -306,6 +358,10 @@ def main():
         log("[bench] batch: 192 systems of the config-1 family ...")
         out["batch_config3a"] = bench_batch()
         log(f"[bench] batch: {out['batch_config3a']}")
+    if "batch3b" in parts:
+        log("[bench] batch: 48 uneven random-block systems ...")
+        out["batch_config3b"] = bench_batch3b()
+        log(f"[bench] batch3b: {out['batch_config3b']}")
     if "unet" in parts:
         log("[bench] warm-start network at 96x96x80 cells ...")
         out["unet"] = bench_unet()
