@@ -62,7 +62,8 @@ def build_fortran() -> str:
         return ""
     bdir = os.path.join(HERE, "fortran", "_mod")
     os.makedirs(bdir, exist_ok=True)
-    srcs = [os.path.join(FSRC, f) for f in ("aphid_gpu_mod.f90", "aphid_driver.f90")]
+    srcs = [os.path.join(FSRC, f) for f in ("aphid_gpu_mod.f90", "aphid_vtk.f90",
+                                            "aphid_driver.f90")]
     if not all(os.path.exists(s) for s in srcs):
         return ""
     exe = os.path.join(HERE, "aphid_gpu_serial")

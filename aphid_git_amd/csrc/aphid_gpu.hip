@@ -18,6 +18,7 @@
 #include "setup.hpp"
 #include "stencil.hpp"
 #include "warm.hpp"
+#include "fields.hpp"
 
 namespace {
 
@@ -1248,6 +1249,48 @@ int aphid_warm_start(aphid_ctx* c, aphid_unet* net, const float* feat, unsigned 
     rel = 1.0;
   }
   if (r0_rel) *r0_rel = rel;
+  return 0;
+}
+
+int aphid_gen_fields(aphid_ctx* c, double omega, double* bx, double* by, double* bz, double* ex,
+                     double* ey, double* ez) {
+  if (!c || !bx || !by || !bz || !ex || !ey || !ez) return fail(APHID_E_ARG, "NULL argument");
+  if (!c->V[0] || !c->dmet) return fail(APHID_E_STATE, "no resident solution");
+  CK(hipSetDevice(c->dev));
+  const long long nn = c->nn;
+  const double* Dx = c->dmet;
+  const double* Dy = Dx + (c->nx + 1);
+  const double* Dz = Dy + (c->ny + 1);
+  hipLaunchKernelGGL(k_fields, dim3(grid_for(nn)), dim3(kBlock), 0, c->s, (const c128*)c->V[0],
+                     dims(c), Dx, Dy, Dz, omega, c->V[7], c->V[8]);
+  CK(hipGetLastError());
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  struct Out { const c128* src; double* h; int n1, n2, n3; };
+  const Out outs[6] = {{c->V[7], bx, nx, ny - 1, nz - 1},     {c->V[7] + nn, by, nx - 1, ny, nz - 1},
+                       {c->V[7] + 2 * nn, bz, nx - 1, ny - 1, nz}, {c->V[8], ex, nx - 1, ny, nz},
+                       {c->V[8] + nn, ey, nx, ny - 1, nz},      {c->V[8] + 2 * nn, ez, nx, ny, nz - 1}};
+  for (const Out& o : outs) {
+    const long long n = (long long)o.n1 * o.n2 * o.n3;
+    hipLaunchKernelGGL(k_d2f<double>, dim3(grid_for(n)), dim3(kBlock), 0, c->s, o.src, o.n1, o.n2,
+                       o.n3, c->stage, dims(c));
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(o.h, c->stage, sizeof(c128) * (size_t)n, hipMemcpyDeviceToHost, c->s));
+    CK(hipStreamSynchronize(c->s));
+  }
+  return 0;
+}
+
+int aphid_get_sigma(aphid_ctx* c, double* sig) {
+  if (!c || !sig) return fail(APHID_E_ARG, "NULL argument");
+  if (!c->sig) return fail(APHID_E_STATE, "aphid_set_cells or aphid_set_sigma first");
+  CK(hipSetDevice(c->dev));
+  const int cx = c->nx - 1, cy = c->ny - 1, cz = c->nz - 1;
+  const long long n = (long long)cx * cy * cz;
+  hipLaunchKernelGGL(k_d2f<double>, dim3(grid_for(n)), dim3(kBlock), 0, c->s, (const c128*)c->sig,
+                     cx, cy, cz, c->stage, Dims{cx, cy, cz, n});
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(sig, c->stage, sizeof(c128) * (size_t)n, hipMemcpyDeviceToHost, c->s));
+  CK(hipStreamSynchronize(c->s));
   return 0;
 }
 

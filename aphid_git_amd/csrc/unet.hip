@@ -361,21 +361,21 @@ int upload_target(aphid_unet* n, const float* tgt, bf16_t* dst) {
 
 void free_net(aphid_unet* n) {
   for (auto& L : n->L) {
-    if (L.wf) hipFree(L.wf);
-    if (L.wt) hipFree(L.wt);
+    if (L.wf) (void)hipFree(L.wf);
+    if (L.wt) (void)hipFree(L.wt);
     L.wf = L.wt = nullptr;
   }
   void* ptrs[] = {n->p, n->g, n->m, n->v, n->gy, n->dwpart, n->bpart, n->lpart, n->lacc, n->stage,
                   n->tgt1, n->ds_in, n->ds_tg};
   for (void* q : ptrs)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   for (auto& b : n->buf)
-    if (b) hipFree(b), b = nullptr;
+    if (b) (void)hipFree(b), b = nullptr;
   for (auto& b : n->dbuf)
-    if (b) hipFree(b), b = nullptr;
-  if (n->e0) hipEventDestroy(n->e0);
-  if (n->e1) hipEventDestroy(n->e1);
-  if (n->s) hipStreamDestroy(n->s);
+    if (b) (void)hipFree(b), b = nullptr;
+  if (n->e0) (void)hipEventDestroy(n->e0);
+  if (n->e1) (void)hipEventDestroy(n->e1);
+  if (n->s) (void)hipStreamDestroy(n->s);
 }
 
 int create(aphid_unet* n, int dev, int nx, int ny, int nz, unsigned long long seed) {

@@ -9,13 +9,16 @@
 ! the FP32 cell parameters on the device (read_mesh, :269-454), builds  !
 ! the right-hand side (gen_rhs, :534-743), and runs the Jacobi-QMR      !
 ! solve (qmr, src/aphid_qmr.f90:5-434) through the C ABI.  Writes       !
-! dir/qmr.iter (iteration, residual norm) as the reference does and the !
+! dir/qmr.iter (iteration, residual norm) as the reference does, the    !
 ! solution (A_x, A_y, A_z, Phi in the reference's shapes) as a raw      !
-! stream file dir/aphid_solution.bin.                                   !
+! stream file dir/aphid_solution.bin, and the reference's VTK files     !
+! (module aphid_vtk): sigma.vtk, aphid_a_phi.vtk, Bx..Ez.vtk and        !
+! EM_grid.vtk from the fields computed on the device (gen_fields).      !
 !----------------------------------------------------------------------!
       program aphid_gpu_serial
       use iso_c_binding
       use aphid_gpu
+      use aphid_vtk
       implicit none
 
       integer, parameter :: dp = 8
@@ -33,6 +36,12 @@
       complex(c_double_complex), allocatable :: bx(:,:,:), by(:,:,:)
       complex(c_double_complex), allocatable :: bz(:,:,:), bp(:,:,:)
       real(c_double), allocatable, target :: hist(:)
+      complex(c_double_complex), allocatable :: sig(:,:,:)
+      complex(c_double_complex), allocatable :: fbx(:,:,:), fby(:,:,:)
+      complex(c_double_complex), allocatable :: fbz(:,:,:), fex(:,:,:)
+      complex(c_double_complex), allocatable :: fey(:,:,:), fez(:,:,:)
+      complex(c_double_complex), allocatable :: k2x(:,:,:), k2y(:,:,:)
+      complex(c_double_complex), allocatable :: k2z(:,:,:), k2p(:,:,:)
       type(c_ptr) :: ctx, pa, pb, pc, pd
       type(aphid_qmr_report) :: rep
       integer :: nblk, b
@@ -114,6 +123,12 @@
       rc = aphid_set_cells(ctx, model, omega, pa, pb, pc, pd)
       call check(rc, 'aphid_set_cells')
 
+!     sigma.vtk right after the model is read (src/aphid.f90:79)
+      allocate(sig(nx-1, ny-1, nz-1))
+      rc = aphid_get_sigma(ctx, sig)
+      call check(rc, 'aphid_get_sigma')
+      call vtk_sigma(trim(wdir)//'/sigma.vtk', xn, yn, zn, sig)
+
 !     sources (mode 5: one "+X i j k" per line until end of file)
       ns = 0
       nsmax = 0
@@ -190,6 +205,20 @@
            access='stream', form='unformatted', status='replace')
       write(15) bx, by, bz, bp
       close(15)
+
+!     post-processing as the reference's aphid_main (src/aphid.f90:110-112)
+      call vtk_a_phi(trim(wdir)//'/aphid_a_phi.vtk', xn, yn, zn,        &
+                     bx, by, bz, bp)
+      allocate(fbx(nx,ny-1,nz-1), fby(nx-1,ny,nz-1), fbz(nx-1,ny-1,nz))
+      allocate(fex(nx-1,ny,nz), fey(nx,ny-1,nz), fez(nx,ny,nz-1))
+      rc = aphid_gen_fields(ctx, omega, fbx, fby, fbz, fex, fey, fez)
+      call check(rc, 'aphid_gen_fields')
+      allocate(k2x(nx-1,ny,nz), k2y(nx,ny-1,nz), k2z(nx,ny,nz-1),       &
+               k2p(nx,ny,nz))
+      rc = aphid_get_k2(ctx, k2x, k2y, k2z, k2p)
+      call check(rc, 'aphid_get_k2')
+      call vtk_fields(trim(wdir), xn, yn, zn, fbx, fby, fbz, fex, fey,   &
+                      fez, k2x, k2y, k2z, omega)
 
       rc = aphid_ctx_destroy(ctx)
 

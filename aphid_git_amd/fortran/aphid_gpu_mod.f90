@@ -89,6 +89,23 @@
           complex(c_double_complex), intent(out) :: kx(*), ky(*), kz(*), kp(*)
         end function
 
+        integer(c_int) function aphid_get_sigma(ctx, sig)              &
+                                bind(C, name='aphid_get_sigma')
+          import :: c_int, c_ptr, c_double_complex
+          type(c_ptr), value :: ctx
+          complex(c_double_complex), intent(out) :: sig(*)
+        end function
+
+        integer(c_int) function aphid_gen_fields(ctx, omega, bx, by,   &
+                                bz, ex, ey, ez)                        &
+                                bind(C, name='aphid_gen_fields')
+          import :: c_int, c_ptr, c_double, c_double_complex
+          type(c_ptr), value :: ctx
+          real(c_double), value :: omega
+          complex(c_double_complex), intent(out) :: bx(*), by(*), bz(*)
+          complex(c_double_complex), intent(out) :: ex(*), ey(*), ez(*)
+        end function
+
         integer(c_int) function aphid_get_diag(ctx, dx, dy, dz, dp)    &
                                 bind(C, name='aphid_get_diag')
           import :: c_int, c_ptr, c_double_complex

@@ -261,6 +261,23 @@ class Solver:
                                       _p(f), C.c_uint(flags), C.byref(r0)), "aphid_warm_start")
         return r0.value
 
+    def gen_fields(self, omega: float):
+        """B = curl A on faces, E = -i omega (A - grad Phi) on edges of the
+        resident solution: [Bx, By, Bz, Ex, Ey, Ez] in the reference's shapes."""
+        nx, ny, nz = self.shape
+        shp = [(nx, ny - 1, nz - 1), (nx - 1, ny, nz - 1), (nx - 1, ny - 1, nz),
+               (nx - 1, ny, nz), (nx, ny - 1, nz), (nx, ny, nz - 1)]
+        out = [np.zeros(s, np.complex128, order="F") for s in shp]
+        _check(lib().aphid_gen_fields(self._ctx, C.c_double(omega), *[_p(a) for a in out]),
+               "aphid_gen_fields")
+        return out
+
+    def get_sigma(self) -> np.ndarray:
+        nx, ny, nz = self.shape
+        out = np.zeros((nx - 1, ny - 1, nz - 1), np.complex128, order="F")
+        _check(lib().aphid_get_sigma(self._ctx, _p(out)), "aphid_get_sigma")
+        return out
+
     def fetch_solution(self):
         out = self._zeros4()
         _check(lib().aphid_fetch_solution(self._ctx, *[_p(a) for a in out]),
@@ -487,4 +504,4 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_unet_forward_flops", "aphid_unet_get_params", "aphid_unet_set_params",
            "aphid_unet_forward", "aphid_unet_loss_grad", "aphid_unet_activation",
            "aphid_unet_reserve", "aphid_unet_add_sample", "aphid_unet_train", "aphid_unet_time",
-           "aphid_warm_start"]
+           "aphid_warm_start", "aphid_gen_fields", "aphid_get_sigma"]

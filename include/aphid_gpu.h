@@ -224,6 +224,17 @@ int aphid_batch_get_solution(aphid_batch *batch, int sys, double *x_x,
  * sys (same mesh), e.g. the previous frequency of a sweep. */
 int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
 
+/* Post-processing of the resident solution (gen_fields, src/aphid.f90:
+ * 2173-2252): B = curl A on the primary cell faces -- Bx (nx,ny-1,nz-1),
+ * By (nx-1,ny,nz-1), Bz (nx-1,ny-1,nz) -- and E = -i omega (A - grad Phi)
+ * on the primary edges -- Ex (nx-1,ny,nz), Ey (nx,ny-1,nz), Ez (nx,ny,nz-1)
+ * -- complex, Fortran order. */
+int aphid_gen_fields(aphid_ctx *ctx, double omega, double *bx, double *by,
+                     double *bz, double *ex, double *ey, double *ez);
+/* The complex sigma-hat of every cell (nx-1,ny-1,nz-1) as the operator
+ * uses it (read_mesh's models, src/aphid.f90:379-449), for dump_sigma. */
+int aphid_get_sigma(aphid_ctx *ctx, double *sig);
+
 /* ---------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference, which always starts
  * QMR from x0 = 0 (src/aphid_qmr.f90:230-233).

@@ -11,6 +11,10 @@
 !   ref_harness qmr1|qmr4    -> ref_qmr_sol.bin (+ the reference's own  !
 !                            qmr.iter); qmr1 = serial qmr, qmr4 =      !
 !                            4-thread qmr_mthread                      !
+!   ref_harness fields       serial qmr, then gen_fields and the VTK    !
+!                            writers: ref_qmr_sol.bin, ref_fields.bin   !
+!                            (Bx,By,Bz,Ex,Ey,Ez), sigma.vtk, Bx..Ez.vtk !
+!                            and ref_a_phi.vtk (dump_a_phi's unit 11)   !
 ! Binary files are raw stream records in the order written below.      !
 !----------------------------------------------------------------------!
       program ref_harness
@@ -81,8 +85,24 @@
              form='unformatted', status='replace')
         write(24) rhs_x, rhs_y, rhs_z, rhs_p
         close(24)
+      else if (trim(cmd) .eq. 'fields') then
+        call dump_sigma
+        call qmr
+        open(unit=24, file='ref_qmr_sol.bin', access='stream',          &
+             form='unformatted', status='replace')
+        write(24) rhs_x, rhs_y, rhs_z, rhs_p
+        close(24)
+!       dump_a_phi writes to unit 11 without opening it
+        open(unit=11, file='ref_a_phi.vtk', status='replace')
+        call dump_a_phi
+        call gen_fields
+        call dump_fields
+        open(unit=25, file='ref_fields.bin', access='stream',           &
+             form='unformatted', status='replace')
+        write(25) Bx, By, Bz, Ex, Ey, Ez
+        close(25)
       else
-        write(6,*) 'usage: ref_harness setup | apply N | qmr1 | qmr4'
+        write(6,*) 'usage: ref_harness setup | apply N | qmr1 | qmr4 | fields'
         call mpi_finalize(ierr)
         stop 2
       end if

@@ -132,6 +132,30 @@ def golden_case(name, case, napply=3, qmr=False, sample=None, setup=True):
     print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
 
 
+VTK_FILES = ("sigma.vtk", "ref_a_phi.vtk", "Bx.vtk", "By.vtk", "Bz.vtk", "Ex.vtk", "Ey.vtk",
+             "Ez.vtk", "EM_grid.vtk")
+
+
+def golden_fields(name, case):
+    """The reference's serial QMR, gen_fields and VTK writers on one case:
+    its solution, the six field arrays (raw, in order Bx By Bz Ex Ey Ez) and
+    the text of every VTK file it writes (dump_a_phi's unit 11 captured as
+    ref_a_phi.vtk)."""
+    with tempfile.TemporaryDirectory() as d:
+        meshgen.write_case(case, d)
+        out = case_arrays(case)
+        run(d, "fields")
+        out["qmr_sol"] = np.frombuffer(open(os.path.join(d, "ref_qmr_sol.bin"), "rb").read(),
+                                       "c16").copy()
+        out["fields"] = np.frombuffer(open(os.path.join(d, "ref_fields.bin"), "rb").read(),
+                                      "c16").copy()
+        for fn in VTK_FILES:
+            out["vtk_" + fn.replace(".", "_")] = np.array(open(os.path.join(d, fn)).read())
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     T = meshgen.tiny_case
@@ -158,7 +182,15 @@ def main():
     golden_case("rhs_mode4", big, napply=0, setup=False)
     # config-1 family: the 32^3 air/half-space model at 10 Hz, rel-res 1e-8
     golden_case("config1_10Hz", meshgen.config1(), napply=0, qmr=True, sample=0.01, setup=False)
+    # post-processing: gen_fields and the VTK writers
+    golden_fields("fields_R_7x6x5", T(7, 6, 5, seed=1, model="R", freq=3.0))
+    golden_fields("fields_C_8x7x6", T(8, 7, 6, seed=12, model="C", freq=300.0))
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "fields":  # only the post-processing goldens
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+        golden_fields("fields_R_7x6x5", meshgen.tiny_case(7, 6, 5, seed=1, model="R", freq=3.0))
+        golden_fields("fields_C_8x7x6", meshgen.tiny_case(8, 7, 6, seed=12, model="C", freq=300.0))
+    else:
+        main()
