@@ -45,7 +45,8 @@ constexpr int kMaxPartBlocks = 1 << 16; // capacity of the partial-sum buffer
 struct Tile {
   int tk, tj, chunk;  // chunk = x-planes per block (march); 0 = direct variant
 };
-const Tile kTiles[] = {{16, 16, 0}, {32, 8, 0}, {64, 4, 0}, {16, 8, 0}, {32, 4, 0}};
+const Tile kTiles[] = {{16, 16, 0}, {32, 8, 0}, {64, 4, 0}, {16, 8, 0}, {32, 4, 0},
+                       {32, 16, 0}, {64, 8, 0}, {16, 32, 0}};
 
 template <typename R>
 hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, cplx<R>* out,
@@ -64,6 +65,9 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
     APHID_TILE(64, 4)
     APHID_TILE(16, 8)
     APHID_TILE(32, 4)
+    APHID_TILE(32, 16)
+    APHID_TILE(64, 8)
+    APHID_TILE(16, 32)
 #undef APHID_TILE
     return hipErrorInvalidValue;
   }
@@ -79,6 +83,9 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
   APHID_TILE(64, 4)
   APHID_TILE(16, 8)
   APHID_TILE(32, 4)
+  APHID_TILE(32, 16)
+  APHID_TILE(64, 8)
+  APHID_TILE(16, 32)
 #undef APHID_TILE
   return hipErrorInvalidValue;
 }
@@ -325,7 +332,7 @@ int autotune(aphid_ctx* c, int prec) {
   }
   int rc = ensure_vectors(c, prec == 1);
   if (rc) return rc;
-  const int chunks[] = {0, 2, 4, 6, 8, 12, 16};  // 0 = direct variant
+  const int chunks[] = {0, 2, 4, 6, 8, 12, 16, 24, 32};  // 0 = direct variant
   float best = 1e30f;
   Tile bt = c->tile[prec];
   const int reps = c->nn > 2000000 ? 5 : 20;

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -123,10 +124,26 @@ int lv_dims(const aphid_unet* n, int level, int& X, int& Y, int& Z) {
   return n->V[level];
 }
 
-#define CONV_CASE(KS, CI, CO, MODE)                                                   \
-  if (ks == KS && cin == CI && cout == CO && mode == MODE) {                          \
-    hipLaunchKernelGGL((kc_conv<KS, CI, CO, MODE>), grid, dim3(256), 0, s, a);        \
-    return hipGetLastError();                                                         \
+// 3x3x3 layers use the LDS-brick kernel (APHID_CONV_DIRECT=1: the
+// global-load kernel, kept as the reference point); 1x1 layers the latter
+bool conv_lds() {
+  static const bool on = [] {
+    const char* e = getenv("APHID_CONV_DIRECT");
+    return !(e && e[0] == '1');
+  }();
+  return on;
+}
+
+#define CONV_CASE(KS, CI, CO, MODE)                                                     \
+  if (ks == KS && cin == CI && cout == CO && mode == MODE) {                            \
+    if (KS == 3 && conv_lds()) {                                                        \
+      using Bk = Brick<CI, CO>;                                                         \
+      const dim3 gb((unsigned)brick_count(a.X, a.Y, a.Z, Bk::TX, Bk::TY));              \
+      hipLaunchKernelGGL((kc_conv3l<CI, CO, MODE>), gb, dim3(256), 0, s, a);            \
+    } else {                                                                            \
+      hipLaunchKernelGGL((kc_conv<KS, CI, CO, MODE>), grid, dim3(256), 0, s, a);        \
+    }                                                                                   \
+    return hipGetLastError();                                                           \
   }
 
 hipError_t launch_conv(int ks, int cin, int cout, int mode, const ConvArgs& a, hipStream_t s) {

@@ -136,6 +136,105 @@ __global__ __launch_bounds__(256) void kc_conv(ConvArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The same 3x3x3 convolution with the input staged in LDS.  A block owns an
+// output brick of TX x TY x 16 voxels (4 waves x MT rows of 16 z-voxels;
+// TX*TY = 4*MT) and loads the brick with a one-voxel halo once, zero outside
+// the grid; every tap's A fragment is then one 16-byte LDS read, so each
+// input value leaves global memory once per block instead of once per tap.
+template <int CIN, int COUT>
+struct Brick {
+  static constexpr int NB = (COUT + 15) / 16;
+  static constexpr int MT = NB >= 8 ? 1 : 8 / NB;
+  static constexpr int TX = MT >= 8 ? 4 : MT >= 4 ? 4 : 2;
+  static constexpr int TY = 4 * MT / TX;
+  static constexpr int BX = TX + 2, BY = TY + 2, BZ = 18;
+  static constexpr int NV = BX * BY * BZ;  // staged voxels
+  static_assert(TX * TY == 4 * MT, "brick shape");
+};
+
+inline int brick_count(int X, int Y, int Z, int tx, int ty) {
+  return ((X + tx - 1) / tx) * ((Y + ty - 1) / ty) * ((Z + 15) / 16);
+}
+
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
+  using Bk = Brick<CIN, COUT>;
+  constexpr int NB = Bk::NB, MT = Bk::MT, TX = Bk::TX, TY = Bk::TY;
+  constexpr int BY = Bk::BY, BZ = Bk::BZ, NV = Bk::NV;
+  constexpr int KPAD = cpad(27 * CIN, 32);
+  constexpr int PCS = CIN / 8;  // 16-byte pieces per voxel
+  static_assert(CIN % 8 == 0, "CIN must be a multiple of 8");
+  __shared__ __attribute__((aligned(16))) bf16_t lds[NV * CIN];
+  const int tid = threadIdx.x, lane = tid % 64, r = lane % 16, g = lane / 16, w = tid / 64;
+  const int nbz = (a.Z + 15) / 16, nby = (a.Y + TY - 1) / TY;
+  const int b = blockIdx.x;
+  const int z0 = (b % nbz) * 16, y0 = ((b / nbz) % nby) * TY, x0 = (b / (nbz * nby)) * TX;
+
+  // stage the brick with its halo (coalesced: pieces of a voxel, then z)
+  for (int p = tid; p < NV * PCS; p += 256) {
+    const int vox = p / PCS, pc = p - vox * PCS;
+    const int lz = vox % BZ, ly = (vox / BZ) % BY, lx = vox / (BZ * BY);
+    const int x = x0 + lx - 1, y = y0 + ly - 1, z = z0 + lz - 1;
+    u32x4 q = {0u, 0u, 0u, 0u};
+    if (x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z)
+      q = *reinterpret_cast<const u32x4*>(a.in + (size_t)((x * a.Y + y) * a.Z + z) * a.in_ld +
+                                          a.in_off + 8 * pc);
+    *reinterpret_cast<u32x4*>(&lds[vox * CIN + 8 * pc]) = q;
+  }
+  __syncthreads();
+
+  f32x4 acc[MT][NB];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int ks = 0; ks < KPAD / 32; ++ks) {
+    const int kk = ks * 32 + 8 * g;
+    const int tap = kk / CIN, ci = kk - tap * CIN;
+    const bool tv = tap < 27;
+    const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
+    bf16x8 bw[NB];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) bw[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + kk);
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      const int row = w * MT + t, tx = row / TY, ty = row - tx * TY;
+      const int vox = ((tx + dx) * BY + (ty + dy)) * BZ + (r + dz);
+      const bf16x8 av = tv ? ld8(&lds[vox * CIN + ci]) : zero8();
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[n], acc[t][n], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int row = w * MT + t, tx = row / TY, ty = row - tx * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= a.X || y >= a.Y) continue;
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      const int co = 16 * n + r;
+      if (co >= COUT) continue;
+      const float bias = MODE == 2 ? 0.f : a.bias[co];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int z = z0 + 4 * g + e;
+        if (z >= a.Z) continue;
+        float val = acc[t][n][e] + bias;
+        const size_t o = (size_t)((x * a.Y + y) * a.Z + z) * a.out_ld + a.out_off + co;
+        if (MODE == 0) {
+          val = fmaxf(val, 0.f);
+          reinterpret_cast<bf16_t*>(a.out)[o] = f2bf(val);
+        } else {
+          reinterpret_cast<float*>(a.out)[o] = val;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: dw[co][kk] = sum_v gy[v][co] * in[v + off(tap(kk))][ci(kk)]
 // A GEMM with K running over voxels.  A block owns 256 columns kk (4 waves x
 // 4 blocks of 16) and one voxel split; each 32-voxel step stages gy and the

@@ -90,7 +90,7 @@ def test_every_tile_shape_and_variant_gives_the_same_results(gpu):
     _setup(gpu, case)
     u = _rand4(case.shape, 7)
     ref = None
-    for tk, tj in [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4)]:
+    for tk, tj in [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4), (32, 16), (64, 8), (16, 32)]:
         for chunk in (0, 1, 3, 8, 64):
             print(f"[tile] {tk}x{tj} chunk {chunk}", flush=True)
             gpu.set_tile(tk, tj, chunk)
