@@ -179,6 +179,12 @@ struct aphid_ctx {
   Tile tile[2] = {{16, 16, 8}, {16, 16, 8}};
   bool tuned[2] = {false, false};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // stagnation stop of a solve (used for the complex64 inner solves of the
+  // mixed-precision refinement): stop when the residual has not fallen below
+  // stall_factor times its best value for stall_window iterations
+  int stall_window = 0;
+  double stall_factor = 0.9;
+  bool stalled = false;
 };
 
 namespace {
@@ -434,9 +440,28 @@ bool use_graphs() {
   return !(e && e[0] == '1');
 }
 
+// host-side stagnation test after each batch of iterations (see
+// aphid_ctx::stall_window); true when the solve should stop
+struct StallTracker {
+  double best = INFINITY;
+  int best_it = 0;
+  bool step(const aphid_ctx* c) {
+    if (c->stall_window <= 0) return false;
+    const double rn = c->hst->rnorm;
+    const int it = c->hst->iter;
+    if (rn < c->stall_factor * best) {
+      best = rn;
+      best_it = it;
+    }
+    return it - best_it >= c->stall_window;
+  }
+};
+
 template <typename V>
 int run_loop(aphid_ctx* c) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
+  c->stalled = false;
+  StallTracker stall;
   if (!use_graphs()) {
     const int k = graph_iters(c, prec);
     const long long max_launch = (long long)c->hst->max_it / k + 3;
@@ -449,6 +474,10 @@ int run_loop(aphid_ctx* c) {
       CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
       CK(hipStreamSynchronize(c->s));
       if (c->hst->done) return 0;
+      if (stall.step(c)) {
+        c->stalled = true;
+        return 0;
+      }
       if (c->hst->iter <= last_iter || l > max_launch)
         return fail(APHID_E_STATE, "QMR state did not advance on the device");
       last_iter = c->hst->iter;
@@ -477,6 +506,10 @@ int run_loop(aphid_ctx* c) {
     CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
     CK(hipStreamSynchronize(c->s));
     if (c->hst->done) break;
+    if (stall.step(c)) {
+      c->stalled = true;
+      break;
+    }
     if (c->hst->iter <= last_iter || l > max_launch)
       return fail(APHID_E_STATE, "QMR state did not advance on the device");
     last_iter = c->hst->iter;
@@ -563,6 +596,8 @@ int solve_mixed(aphid_ctx* c, double tol, int max_it, bool have_x0, QmrState* ou
   const long long n4 = 4 * c->nn;
   if (!have_x0) CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
   const double floor_tol = mixed_inner_floor();
+  const char* sw = getenv("APHID_MIXED_STALL");
+  c->stall_window = sw ? std::max(0, atoi(sw)) : 0;
   int total = 0, sweeps = 0, status = ST_MAXIT;
   double rel = NAN, rn = NAN, bn = NAN;
   QmrState last{};
@@ -586,9 +621,13 @@ int solve_mixed(aphid_ctx* c, double tol, int max_it, bool have_x0, QmrState* ou
     const double inner_tol = std::max(0.5 * tol * bn / rn, floor_tol);
     double s_in = 0.0;
     rc = qmr_device<float>(c, inner_tol, std::max(max_it - total, 1), false, &s_in);
-    if (rc) return rc;
+    if (rc) {
+      c->stall_window = 0;
+      return rc;
+    }
     last = *c->hst;
-    const int executed = last.status == ST_MAXIT ? std::max(max_it - total, 1) : last.iter;
+    const int executed = c->stalled ? std::max(last.iter - 1, 1)
+                         : last.status == ST_MAXIT ? std::max(max_it - total, 1) : last.iter;
     total += std::max(executed, 0);
     if (last.status == ST_BREAK && executed <= 1) {
       status = ST_BREAK;
@@ -599,6 +638,8 @@ int solve_mixed(aphid_ctx* c, double tol, int max_it, bool have_x0, QmrState* ou
     CK(hipGetLastError());
     ++sweeps;
   }
+  c->stall_window = 0;
+  c->stalled = false;
   CK(hipStreamSynchronize(c->s));
   *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out = last;

@@ -47,7 +47,7 @@ BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
 PARTS = ("solve", "batch", "unet", "warm", "cpu")
-EXTRA_PARTS = ("batch3b",)  # not in the default set (about a minute of GPU time)
+EXTRA_PARTS = ("batch3b", "warm5")  # not in the default set (a minute or more of GPU time)
 
 
 def log(msg):
@@ -208,6 +208,11 @@ def bench_batch3b(max_it: int = 10000):
                               for q, n in enumerate((64, 96, 128))}}
 
 
+def bench_warm5(ntrain=128, steps=1500, neval=16):
+    """The learned warm start at its full cell grid (96x96x80 cells)."""
+    return bench_warm(ncell=(96, 96, 80), ntrain=ntrain, steps=steps, neval=neval)
+
+
 def bench_unet(reps: int = 5):
     from aphid_git_amd import gpu
 
@@ -358,6 +363,10 @@ def main():
         log("[bench] batch: 192 systems of the config-1 family ...")
         out["batch_config3a"] = bench_batch()
         log(f"[bench] batch: {out['batch_config3a']}")
+    if "warm5" in parts:
+        log("[bench] learned warm start, 96x96x80 cells ...")
+        out["warm_start_96x96x80"] = bench_warm5()
+        log(f"[bench] warm5: {out['warm_start_96x96x80']}")
     if "batch3b" in parts:
         log("[bench] batch: 48 uneven random-block systems ...")
         out["batch_config3b"] = bench_batch3b()
