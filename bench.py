@@ -340,7 +340,10 @@ def main():
         tf = os.path.join(ROOT, "profiles", "traffic_config2.json")
         if os.path.exists(tf):
             with open(tf) as f:
-                traffic = json.load(f).get("stencil_bytes_per_launch")
+                tj = json.load(f)
+            # PMC traffic of the same kernel and tile only
+            if list(tj.get("tile", [])) == list(tile):
+                traffic = tj.get("stencil_bytes_per_launch")
         s.close()
         out.update({
             "value": converged / elapsed,
