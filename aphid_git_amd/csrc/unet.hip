@@ -177,10 +177,18 @@ hipError_t launch_conv(int ks, int cin, int cout, int mode, const ConvArgs& a, h
 }
 #undef CONV_CASE
 
-#define DW_CASE(KS, CI, CO)                                                            \
-  if (ks == KS && cin == CI && cout == CO) {                                           \
-    hipLaunchKernelGGL((kc_dw<KS, CI, CO>), grid, dim3(256), 0, s, a);                 \
-    return hipGetLastError();                                                          \
+// 3x3x3 layers: one block per LDS brick (dw_splits gives their count);
+// 1x1 layers (and APHID_CONV_DIRECT=1): the voxel-split global-gather kernel
+#define DW_CASE(KS, CI, CO)                                                              \
+  if (ks == KS && cin == CI && cout == CO) {                                             \
+    if constexpr (KS == 3) {                                                             \
+      if (conv_lds()) {                                                                  \
+        hipLaunchKernelGGL((kc_dw3l<CI, CO>), dim3(grid.y), dim3(256), 0, s, a, a.vper);  \
+        return hipGetLastError();                                                        \
+      }                                                                                  \
+    }                                                                                    \
+    hipLaunchKernelGGL((kc_dw<KS, CI, CO>), grid, dim3(256), 0, s, a);                   \
+    return hipGetLastError();                                                            \
   }
 hipError_t launch_dw(int ks, int cin, int cout, dim3 grid, const DwArgs& a, hipStream_t s) {
   DW_CASE(3, 8, 16)
@@ -422,12 +430,20 @@ int create(aphid_unet* n, int dev, int nx, int ny, int nz, unsigned long long se
     L.boff = off;
     off += L.sp.cout;
     const int V = n->V[L.sp.level];
-    const int ntile = (L.kpad + 255) / 256;
-    const int want = std::max(1, 512 / ntile);  // ~2 blocks per CU
-    const int maxs = std::max(1, (V + 31) / 32);
-    int ns = std::min(want, maxs);
-    L.vper = cpad((V + ns - 1) / ns, 32);
-    L.nsplit = (V + L.vper - 1) / L.vper;
+    if (L.sp.ks == 3 && conv_lds()) {  // blocks of LDS bricks, one partial per block
+      const int nvox = L.sp.cinp <= 16 ? 512 : L.sp.cinp <= 32 ? 256 : 128;
+      const int tx = nvox == 128 ? 2 : 4, ty = nvox / 16 / tx;
+      const int nbr = brick_count(n->X[L.sp.level], n->Y[L.sp.level], n->Z[L.sp.level], tx, ty);
+      L.nsplit = std::min(nbr, 512);
+      L.vper = nbr;  // the kernel's brick count
+    } else {
+      const int ntile = (L.kpad + 255) / 256;
+      const int want = std::max(1, 512 / ntile);  // ~2 blocks per CU
+      const int maxs = std::max(1, (V + 31) / 32);
+      int ns = std::min(want, maxs);
+      L.vper = cpad((V + ns - 1) / ns, 32);
+      L.nsplit = (V + L.vper - 1) / L.vper;
+    }
     part_max = std::max(part_max, (size_t)L.nsplit * L.coutp * L.kpad);
     gy_max = std::max(gy_max, (size_t)V * L.sp.cout);
     UCK(hipMalloc(&L.wf, sizeof(bf16_t) * L.coutp * L.kpad));

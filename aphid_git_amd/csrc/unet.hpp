@@ -352,6 +352,143 @@ __global__ __launch_bounds__(256) void kc_dw(DwArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Weight gradient of a 3x3x3 layer from LDS bricks of TX x TY x 16 output
+// voxels (NVOX = 512, 256 or 128 for CIN <= 16, 32, 64).  Block b
+// accumulates bricks b, b + gridDim, ... in registers (a wave holds all of
+// its output columns: n-blocks w, w+4, ...) and writes one partial dw to
+// part[b].  For each brick it stages
+//   gt[co][vox]          dY of its voxels, transposed (voxel-contiguous);
+//   bt[ci][bx][by][bz]   its input brick with a one-voxel halo, transposed
+//                        (z-contiguous, rows padded to 24);
+// so that both matrix-core operands are 16-byte LDS reads of 8 consecutive
+// voxels.  K runs over the block's voxels in steps of 32 = two 16-voxel rows;
+// a lane's 8 voxels are z0..z0+7 (z0 = 0 or 8) of one row, whose inputs for
+// a tap with z offset dz are bt[...][z0+dz .. z0+dz+7]: two aligned reads and
+// a 16- or 32-bit funnel shift.
+template <int CIN>
+struct DwBrick {
+  static constexpr int NVOX = CIN <= 16 ? 512 : CIN <= 32 ? 256 : 128;
+  static constexpr int TX = NVOX == 512 ? 4 : NVOX == 256 ? 4 : 2;
+  static constexpr int TY = NVOX / 16 / TX;
+  static constexpr int BX = TX + 2, BY = TY + 2, BZP = 24;
+};
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void kc_dw3l(DwArgs a, int nbricks) {
+  using Bk = DwBrick<CIN>;
+  constexpr int NVOX = Bk::NVOX, TX = Bk::TX, TY = Bk::TY, BX = Bk::BX, BY = Bk::BY;
+  constexpr int BZP = Bk::BZP, ROW = 18;  // staged z extent of a brick row
+  constexpr int KTOT = 27 * CIN, KPAD = cpad(KTOT, 32);
+  constexpr int COUTP = cpad(COUT, 16), MB = COUTP / 16;
+  constexpr int NBN = KPAD / 16;            // n-blocks of 16 columns
+  constexpr int NPW = (NBN + 3) / 4;        // n-blocks of one wave (w, w+4, ...)
+  constexpr int NCMAX = 64 / MB;            // accumulators <= 256 per lane
+  constexpr int NC = NPW < NCMAX ? NPW : NCMAX;
+  constexpr int NPASS = (NPW + NC - 1) / NC;
+  constexpr int BPL = BX * BY * BZP;        // elements per channel of the brick
+  static_assert(CIN % 8 == 0 && COUT % 8 == 0, "channel counts");
+  __shared__ __attribute__((aligned(16))) bf16_t gt[COUTP * NVOX];
+  __shared__ __attribute__((aligned(16))) bf16_t bt[CIN * BPL];
+  const int tid = threadIdx.x, lane = tid % 64, r = lane % 16, g = lane / 16, w = tid / 64;
+  const int nbz = (a.Z + 15) / 16, nby = (a.Y + TY - 1) / TY;
+  float* P = a.part + (size_t)blockIdx.x * COUTP * KPAD;
+
+  for (int pass = 0; pass < NPASS; ++pass) {
+    f32x4 acc[MB][NC];
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[m][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int br = blockIdx.x; br < nbricks; br += gridDim.x) {
+      const int z0b = (br % nbz) * 16, y0b = ((br / nbz) % nby) * TY, x0b = (br / (nbz * nby)) * TX;
+      // dY of the brick's voxels, transposed; zero outside the grid and for
+      // padded output channels
+      for (int p = tid; p < NVOX * (COUTP / 8); p += 256) {
+        const int vox = p % NVOX, pc = p / NVOX;
+        const int row = vox / 16, z = z0b + vox % 16, x = x0b + row / TY, y = y0b + row % TY;
+        u32x4 q = {0u, 0u, 0u, 0u};
+        if (8 * pc < COUT && x < a.X && y < a.Y && z < a.Z)
+          q = *reinterpret_cast<const u32x4*>(a.gy + (size_t)((x * a.Y + y) * a.Z + z) * COUT + 8 * pc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          gt[(8 * pc + 2 * e) * NVOX + vox] = (bf16_t)(q[e] & 0xffffu);
+          gt[(8 * pc + 2 * e + 1) * NVOX + vox] = (bf16_t)(q[e] >> 16);
+        }
+      }
+      // the input brick with its halo, transposed; zero outside the grid
+      constexpr int PCS = CIN / 8;
+      for (int p = tid; p < BX * BY * ROW * PCS; p += 256) {
+        const int pc = p % PCS, vox = p / PCS;
+        const int lz = vox % ROW, ly = (vox / ROW) % BY, lx = vox / (ROW * BY);
+        const int x = x0b + lx - 1, y = y0b + ly - 1, z = z0b + lz - 1;
+        u32x4 q = {0u, 0u, 0u, 0u};
+        if (x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z)
+          q = *reinterpret_cast<const u32x4*>(a.in + (size_t)((x * a.Y + y) * a.Z + z) * a.in_ld +
+                                              a.in_off + 8 * pc);
+        const int base = (lx * BY + ly) * BZP + lz;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          bt[(8 * pc + 2 * e) * BPL + base] = (bf16_t)(q[e] & 0xffffu);
+          bt[(8 * pc + 2 * e + 1) * BPL + base] = (bf16_t)(q[e] >> 16);
+        }
+      }
+      __syncthreads();
+
+      for (int ks = 0; ks < NVOX / 32; ++ks) {
+        const int v8 = ks * 32 + 8 * g;  // first of this lane's 8 voxels
+        const int row = v8 / 16, z0 = v8 % 16, tx = row / TY, ty = row % TY;
+        bf16x8 ag[MB];
+#pragma unroll
+        for (int m = 0; m < MB; ++m) ag[m] = ld8(&gt[(16 * m + r) * NVOX + v8]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const int nb = w + 4 * (pass * NC + c);
+          const int col = 16 * nb + r;
+          const int tap = col / CIN, ci = col - tap * CIN;
+          bf16x8 bv = zero8();
+          if (nb < NBN && tap < 27) {
+            const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
+            const bf16_t* src = &bt[ci * BPL + ((tx + dx) * BY + (ty + dy)) * BZP + z0];
+            const u32x4 q0 = *reinterpret_cast<const u32x4*>(src);
+            const u32x4 q1 = *reinterpret_cast<const u32x4*>(src + 8);
+            u32x4 q;
+            if (dz == 0) {
+              q = q0;
+            } else if (dz == 1) {
+              q[0] = __builtin_amdgcn_alignbit(q0[1], q0[0], 16);
+              q[1] = __builtin_amdgcn_alignbit(q0[2], q0[1], 16);
+              q[2] = __builtin_amdgcn_alignbit(q0[3], q0[2], 16);
+              q[3] = __builtin_amdgcn_alignbit(q1[0], q0[3], 16);
+            } else {
+              q[0] = q0[1];
+              q[1] = q0[2];
+              q[2] = q0[3];
+              q[3] = q1[0];
+            }
+            bv = __builtin_bit_cast(bf16x8, q);
+          }
+#pragma unroll
+          for (int m = 0; m < MB; ++m)
+            acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ag[m], bv, acc[m][c], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int nb = w + 4 * (pass * NC + c);
+        if (nb >= NBN) continue;
+        const int kk = 16 * nb + r;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) P[(size_t)(16 * m + 4 * g + e) * KPAD + kk] = acc[m][c][e];
+      }
+  }
+}
+
 // grad[(co*cin_real + ci)*NT + tap] += sum_s part[s][co][tap*cinp + ci]
 // A block of 256 threads owns 64 consecutive outputs; the 4 threads of an
 // output sum the splits congruent to their index mod 4, then combine in a

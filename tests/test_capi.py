@@ -41,3 +41,24 @@ def test_no_gpu_is_an_error_code_not_a_crash():
         pytest.skip("a GPU is visible")
     with pytest.raises(gpu.AphidError):
         gpu.Solver()
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device():
+    """Argument checks of the network, warm-start and post-processing entry
+    points return error codes before touching a device."""
+    import ctypes as C
+
+    from aphid_git_amd import gpu
+
+    if not os.path.exists(gpu.LIB_PATH):
+        pytest.skip("libaphid_gpu.so not built")
+    lib = gpu.lib()
+    with pytest.raises(gpu.AphidError, match="multiples of 4"):
+        gpu.UNet((6, 8, 8))
+    assert lib.aphid_unet_param_count(None) == -1
+    assert lib.aphid_unet_forward(None, None, None) == -1
+    assert lib.aphid_unet_train(None, 1, 1, C.c_float(1e-3), C.c_ulonglong(0), None, None) == -1
+    assert lib.aphid_warm_start(None, None, None, C.c_uint(0), None) == -1
+    assert lib.aphid_gen_fields(None, C.c_double(1.0), None, None, None, None, None, None) == -1
+    assert lib.aphid_get_sigma(None, None) == -1
+    assert b"NULL" in lib.aphid_last_error() or b"argument" in lib.aphid_last_error()
