@@ -56,20 +56,32 @@ def build_hip(verbose_resources: bool = False) -> str:
     return out
 
 
+FPROGRAMS = (("aphid_gpu_serial", ("aphid_vtk.f90", "aphid_driver.f90")),
+             ("aphid_gpu_parallel", ("aphid_parallel_driver.f90",)))
+
+
 def build_fortran() -> str:
-    """Fortran driver: iso_c_binding module + the aphid_gpu_serial program."""
+    """Fortran drivers: the iso_c_binding module, then the aphid_gpu_serial
+    (one case) and aphid_gpu_parallel (multi-frequency / multi-source job)
+    programs.  Returns the serial program's path."""
     if not os.path.exists(FSRC) or not os.path.exists(FC):
         return ""
     bdir = os.path.join(HERE, "fortran", "_mod")
     os.makedirs(bdir, exist_ok=True)
-    srcs = [os.path.join(FSRC, f) for f in ("aphid_gpu_mod.f90", "aphid_vtk.f90",
-                                            "aphid_driver.f90")]
-    if not all(os.path.exists(s) for s in srcs):
+    mod = os.path.join(FSRC, "aphid_gpu_mod.f90")
+    if not os.path.exists(mod):
         return ""
-    exe = os.path.join(HERE, "aphid_gpu_serial")
-    _run([FC, "-O2", "-J" + bdir, *srcs, "-o", exe, "-L" + HERE, "-laphid_gpu",
-          "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
-    return exe
+    mobj = os.path.join(bdir, "aphid_gpu_mod.o")
+    _run([FC, "-O2", "-J" + bdir, "-c", mod, "-o", mobj])
+    link = ["-L" + HERE, "-laphid_gpu", "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib",
+            "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    for name, files in FPROGRAMS:
+        srcs = [os.path.join(FSRC, f) for f in files]
+        if not all(os.path.exists(s) for s in srcs):
+            continue
+        _run([FC, "-O2", "-J" + bdir, "-I" + bdir, mobj, *srcs, "-o",
+              os.path.join(HERE, name), *link])
+    return os.path.join(HERE, "aphid_gpu_serial")
 
 
 def main():

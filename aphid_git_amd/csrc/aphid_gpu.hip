@@ -442,6 +442,28 @@ bool use_graphs() {
 
 // host-side stagnation test after each batch of iterations (see
 // aphid_ctx::stall_window); true when the solve should stop
+// APHID_PROGRESS=<n>: the solve loop reports its iteration count and
+// residual to stderr each time it passes a multiple of n iterations
+int progress_every() {
+  static const int n = [] {
+    const char* e = getenv("APHID_PROGRESS");
+    return e ? std::max(0, atoi(e)) : 0;
+  }();
+  return n;
+}
+
+void report_progress(const aphid_ctx* c, int& next) {
+  const int n = progress_every();
+  if (n <= 0) return;
+  if (next <= 0) next = n;
+  if (c->hst->iter >= next) {
+    fprintf(stderr, "[aphid] iteration %d residual %.3e (target %.3e)\n", c->hst->iter,
+            c->hst->rnorm, c->hst->target);
+    fflush(stderr);
+    while (next <= c->hst->iter) next += n;
+  }
+}
+
 struct StallTracker {
   double best = INFINITY;
   int best_it = 0;
@@ -462,6 +484,7 @@ int run_loop(aphid_ctx* c) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
   c->stalled = false;
   StallTracker stall;
+  int next_report = 0;
   if (!use_graphs()) {
     const int k = graph_iters(c, prec);
     const long long max_launch = (long long)c->hst->max_it / k + 3;
@@ -474,6 +497,7 @@ int run_loop(aphid_ctx* c) {
       CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
       CK(hipStreamSynchronize(c->s));
       if (c->hst->done) return 0;
+      report_progress(c, next_report);
       if (stall.step(c)) {
         c->stalled = true;
         return 0;
@@ -506,6 +530,7 @@ int run_loop(aphid_ctx* c) {
     CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
     CK(hipStreamSynchronize(c->s));
     if (c->hst->done) break;
+    report_progress(c, next_report);
     if (stall.step(c)) {
       c->stalled = true;
       break;

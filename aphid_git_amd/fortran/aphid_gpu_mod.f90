@@ -24,6 +24,11 @@
         integer(c_int) :: outer_iters = 0, reserved = 0
       end type aphid_qmr_report
 
+      type, bind(C) :: aphid_sys_report
+        integer(c_int) :: iters, status
+        real(c_double) :: rnorm, rhsnorm, true_rel_res
+      end type aphid_sys_report
+
       interface
         integer(c_int) function aphid_ctx_create(ctx, device)          &
                                 bind(C, name='aphid_ctx_create')
@@ -166,6 +171,61 @@
                          xp) bind(C, name='aphid_fetch_solution')
           import :: c_int, c_ptr, c_double_complex
           type(c_ptr), value :: ctx
+          complex(c_double_complex), intent(out) :: xx(*), xy(*), xz(*), xp(*)
+        end function
+
+        integer(c_int) function aphid_batch_create(batch, device)      &
+                                bind(C, name='aphid_batch_create')
+          import :: c_int, c_ptr
+          type(c_ptr), intent(out) :: batch
+          integer(c_int), value :: device
+        end function
+
+        integer(c_int) function aphid_batch_destroy(batch)             &
+                                bind(C, name='aphid_batch_destroy')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: batch
+        end function
+
+        integer(c_int) function aphid_batch_add_operator(batch, nx, ny,&
+                                nz, xn, yn, zn, model, omega, a, b, c, d)&
+                                bind(C, name='aphid_batch_add_operator')
+          import :: c_int, c_ptr, c_double, c_char
+          type(c_ptr), value :: batch
+          integer(c_int), value :: nx, ny, nz
+          real(c_double), intent(in) :: xn(*), yn(*), zn(*)
+          character(kind=c_char), value :: model
+          real(c_double), value :: omega
+          type(c_ptr), value :: a, b, c, d
+        end function
+
+        integer(c_int) function aphid_batch_add_system(batch, op, bx,  &
+                                by, bz, bp, x0x, x0y, x0z, x0p)        &
+                                bind(C, name='aphid_batch_add_system')
+          import :: c_int, c_ptr, c_double_complex
+          type(c_ptr), value :: batch
+          integer(c_int), value :: op
+          complex(c_double_complex), intent(in) :: bx(*), by(*), bz(*), bp(*)
+          type(c_ptr), value :: x0x, x0y, x0z, x0p
+        end function
+
+        integer(c_int) function aphid_batch_solve(batch, tol, max_it,  &
+                                flags, reports, seconds)               &
+                                bind(C, name='aphid_batch_solve')
+          import :: c_int, c_ptr, c_double, aphid_sys_report
+          type(c_ptr), value :: batch
+          real(c_double), value :: tol
+          integer(c_int), value :: max_it, flags
+          type(aphid_sys_report), intent(out) :: reports(*)
+          real(c_double), intent(out) :: seconds
+        end function
+
+        integer(c_int) function aphid_batch_get_solution(batch, sys,   &
+                                xx, xy, xz, xp)                        &
+                                bind(C, name='aphid_batch_get_solution')
+          import :: c_int, c_ptr, c_double_complex
+          type(c_ptr), value :: batch
+          integer(c_int), value :: sys
           complex(c_double_complex), intent(out) :: xx(*), xy(*), xz(*), xp(*)
         end function
       end interface

@@ -47,7 +47,7 @@ BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
 PARTS = ("solve", "batch", "unet", "warm", "cpu")
-EXTRA_PARTS = ("batch3b", "warm5")  # not in the default set (a minute or more of GPU time)
+EXTRA_PARTS = ("batch3b", "warm5", "prec4")  # not in the default set (minutes of GPU time)
 
 
 def log(msg):
@@ -213,6 +213,42 @@ def bench_warm5(ntrain=128, steps=1500, neval=16):
     return bench_warm(ncell=(96, 96, 80), ntrain=ntrain, steps=steps, neval=neval)
 
 
+def bench_prec4(freq: float = 1e4, max_it: int = 100000, stall: int = 1000):
+    """Config 4 (192x192x160 cells, 23.9M unknowns, contrast 1e5) at one
+    frequency: the complex128 solve against mixed precision (complex64
+    stencil and vectors, FP64 scalars and refinement; inner solves stop on
+    stagnation) with the same iteration cap."""
+    from aphid_git_amd import gpu, meshgen
+
+    os.environ.setdefault("APHID_PROGRESS", "5000")  # read once, by the first solve
+    case = meshgen.config4(freq=freq)
+    s = gpu.Solver(0)
+    b = s.setup_case(case)
+    out = {"workload": f"config4: 192x192x160-cell stretched mesh, 40 blocks, log10 sigma in [-5, 0], "
+                       f"+X dipole, {freq:g} Hz, rel-res 1e-8", "unknowns": case.n_unknowns,
+           "max_it": max_it}
+    s.load_rhs(b)
+    t0 = time.perf_counter()
+    r = s.solve_resident(TOL, max_it)
+    out["c128"] = {"seconds": time.perf_counter() - t0, "iters": r.iters, "status": r.status,
+                   "true_rel_res": r.true_rel_res}
+    log(f"[bench] prec4 c128: {out['c128']}")
+    x128 = np.concatenate([a.ravel() for a in s.fetch_solution()])
+    os.environ["APHID_MIXED_STALL"] = str(stall)
+    s.load_rhs(b)
+    t0 = time.perf_counter()
+    r = s.solve_resident(TOL, max_it, gpu.F_MIXED)
+    del os.environ["APHID_MIXED_STALL"]
+    x = np.concatenate([a.ravel() for a in s.fetch_solution()])
+    out["mixed"] = {"seconds": time.perf_counter() - t0, "iters_c64": r.iters,
+                    "sweeps": r.outer_iters, "status": r.status, "true_rel_res": r.true_rel_res,
+                    "field_rel_diff_vs_c128": float(np.linalg.norm(x - x128) / np.linalg.norm(x128)),
+                    "stall_window": stall}
+    log(f"[bench] prec4 mixed: {out['mixed']}")
+    s.close()
+    return out
+
+
 def bench_unet(reps: int = 5):
     from aphid_git_amd import gpu
 
@@ -366,6 +402,9 @@ def main():
         log("[bench] batch: 192 systems of the config-1 family ...")
         out["batch_config3a"] = bench_batch()
         log(f"[bench] batch: {out['batch_config3a']}")
+    if "prec4" in parts:
+        log("[bench] precision study on config 4 ...")
+        out["precision_config4"] = bench_prec4()
     if "warm5" in parts:
         log("[bench] learned warm start, 96x96x80 cells ...")
         out["warm_start_96x96x80"] = bench_warm5()

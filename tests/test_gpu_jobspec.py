@@ -47,3 +47,48 @@ def test_job_systems_match_single_solves(tmp_path, stype):
         assert int(lines[3].split()[0]) == mode
         assert float(lines[6].split()[0]) == pytest.approx(f, rel=1e-6)
     s.close()
+
+
+PAR_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "aphid_git_amd", "aphid_gpu_parallel")
+
+
+@pytest.mark.parametrize("stype", ["CS", "MT"])
+def test_fortran_job_driver_matches_run_job(tmp_path, stype):
+    """aphid_gpu_parallel (Fortran over iso_c_binding) writes the same
+    per-process directories as run_job: the same batched solve, so the same
+    iteration counts and the same solutions."""
+    import subprocess
+    if not os.path.exists(PAR_EXE):
+        pytest.fail("aphid_gpu_parallel is not built")
+    arrays = [[(1, 7, 7, 4)], [(-3, 6, 7, 4), (2, 6, 8, 4)]] if stype == "CS" else []
+    freqs = [0.5, 50.0] if stype == "CS" else [2.0, 200.0, 2e4]
+    d, job = _job(tmp_path, stype, freqs, arrays)
+    py_out = str(tmp_path / "py")
+    reports, _, _ = jobspec.run_job(d, out_dir=py_out)
+    r = subprocess.run([PAR_EXE, "-w", d], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n = len(job.systems())
+    assert f"Solved {n:6d} systems" in r.stdout
+    for p, (mode, f, dips) in enumerate(job.systems()):
+        name = jobspec.tmp_dir_name(p, n)
+        fd, pd = os.path.join(d, name), os.path.join(py_out, name)
+        xf = np.fromfile(os.path.join(fd, "aphid_solution.bin"), np.complex128)
+        xp = np.fromfile(os.path.join(pd, "aphid_solution.bin"), np.complex128)
+        assert xf.shape == xp.shape
+        assert np.linalg.norm(xf - xp) <= 1e-12 * np.linalg.norm(xp), p
+        it = open(os.path.join(fd, "qmr.iter")).read().split()
+        assert int(it[0]) == reports[p]["iters"] and int(it[3]) == 0
+        assert float(it[2]) <= 1e-10
+        inp = open(os.path.join(fd, "aphid.input")).read().splitlines()
+        assert int(inp[3].split()[0]) == mode
+        assert float(inp[6].split()[0]) == pytest.approx(f, rel=1e-6)
+        assert inp[4].split()[0] == repr(job.tol) and int(inp[5].split()[0]) == job.max_it
+        assert inp[7].strip().startswith("R")
+        src = os.path.join(fd, "aphid.source")
+        if stype == "CS":
+            got = [ln.split() for ln in open(src).read().splitlines()]
+            assert got == [[meshgen.DIR_NAMES[dr], str(i), str(j), str(k)]
+                           for (dr, i, j, k) in dips]
+        else:
+            assert not os.path.exists(src)
