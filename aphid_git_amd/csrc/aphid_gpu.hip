@@ -45,6 +45,14 @@ constexpr int kMaxPartBlocks = 1 << 16; // capacity of the partial-sum buffer
 struct Tile {
   int tk, tj, chunk;  // chunk = x-planes per block (march); 0 = direct variant
 };
+// Launch shape of a streaming iteration kernel: elements unrolled per thread
+// and step, grid size (= number of reduction partials) and the
+// non-temporal hint.
+struct Stream {
+  int u, grid, nt;
+};
+enum { SK_PUPDATE = 0, SK_VTNORM = 1, SK_UPDATE = 2, SK_COUNT = 3 };
+constexpr Stream kStreamDefault[SK_COUNT] = {{4, 2048, 0}, {4, 2048, 0}, {2, 2048, 0}};
 const Tile kTiles[] = {{16, 16, 0}, {32, 8, 0}, {64, 4, 0}, {16, 8, 0}, {32, 4, 0},
                        {32, 16, 0}, {64, 8, 0}, {16, 32, 0}};
 
@@ -178,6 +186,11 @@ struct aphid_ctx {
   // stencil tiles per precision
   Tile tile[2] = {{16, 16, 8}, {16, 16, 8}};
   bool tuned[2] = {false, false};
+  // launch shapes of the streaming iteration kernels per precision
+  Stream stream[2][SK_COUNT] = {{kStreamDefault[0], kStreamDefault[1], kStreamDefault[2]},
+                                {kStreamDefault[0], kStreamDefault[1], kStreamDefault[2]}};
+  bool stuned[2] = {false, false};
+  QmrState* tst = nullptr;  // scratch QMR state for timing the streaming kernels
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // stagnation stop of a solve (used for the complex64 inner solves of the
   // mixed-precision refinement): stop when the residual has not fallen below
@@ -208,6 +221,7 @@ void free_mesh(aphid_ctx* c) {
   c->stagef = nullptr; c->hist = nullptr; c->hist_cap = 0;
   c->have_k2 = c->have_diag = false;
   c->tuned[0] = c->tuned[1] = false;
+  c->stuned[0] = c->stuned[1] = false;
 }
 
 unsigned grid_for(long long n) {
@@ -401,26 +415,235 @@ const cplx<double>* invd_of<double>(aphid_ctx* c) { return c->invd; }
 template <>
 const cplx<float>* invd_of<float>(aphid_ctx* c) { return c->invdf; }
 
+// The three streaming kernels of an iteration, each with its finalize, in
+// the launch shape `sh` (the grid is also the number of partials).
+#define APHID_STREAM_SHAPES(X) X(2, false) X(2, true) X(4, false) X(4, true)
+
+template <typename V>
+hipError_t launch_pupdate(const Stream& sh, const cplx<V>* v, const cplx<V>* invd, cplx<V>* p,
+                          long long n, const QmrState* st, hipStream_t s) {
+#define X(U, NT)                                                                              \
+  if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
+    hipLaunchKernelGGL((k_pupdate<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, v, invd, p, n, \
+                       st);                                                                   \
+    return hipGetLastError();                                                                 \
+  }
+  APHID_STREAM_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+template <typename V>
+hipError_t launch_vtnorm(const Stream& sh, const cplx<V>* t, const cplx<V>* v, long long n,
+                         QmrState* st, double* part, hipStream_t s) {
+#define X(U, NT)                                                                              \
+  if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
+    hipLaunchKernelGGL((k_vtnorm<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, t, v, n, st,    \
+                       part);                                                                 \
+    hipLaunchKernelGGL(k_fin_vtnorm, dim3(1), dim3(kBlock), 0, s, part, sh.grid, st);          \
+    return hipGetLastError();                                                                 \
+  }
+  APHID_STREAM_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+template <typename V>
+hipError_t launch_update(const Stream& sh, const Vecs<V>& w, const cplx<V>* invd, long long n,
+                         QmrState* st, double* part, double* hist, hipStream_t s) {
+#define X(U, NT)                                                                              \
+  if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
+    hipLaunchKernelGGL((k_update<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, w.p, w.d, w.x,  \
+                       w.t, w.s, w.r, w.v, invd, n, st, part);                                \
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, s, part, sh.grid, st, hist);    \
+    return hipGetLastError();                                                                 \
+  }
+  APHID_STREAM_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+// APHID_TUNE_VERBOSE=1 prints every timed stream-shape candidate to stderr
+bool tune_verbose() {
+  static const bool v = [] {
+    const char* e = getenv("APHID_TUNE_VERBOSE");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
+// Stream-shape tuner.  The streaming kernels interact through the caches:
+// a kernel that reads t and v with plain loads leaves them in the L2 and
+// the Infinity Cache for the next kernel, and the hint on the p-update's
+// stores changes how fast the stencil then reads p.  So the tuner times the
+// iteration's own kernel sequence (p-update, stencil with eps, ||t - beta
+// v||, fused update), each kernel with its finalize, and picks the shapes
+// by coordinate descent on the sequence total.  The tuned set replaces the
+// defaults only when the total is at least 3% lower, so that near-ties do
+// not flip the launch shapes (and with them the order of the reductions)
+// from run to run.  Timing runs on vectors that the solve overwrites
+// before it reads them (v p t d s r and the spare, which stands in for x),
+// with a scratch state reloaded before every launch.  Results are cached
+// with the stencil tiles in APHID_TUNE_CACHE as lines
+// "S nx ny nz prec u grid nt u grid nt u grid nt" (p-update, ||t - beta v||,
+// fused update).
+template <typename V>
+int tune_streams(aphid_ctx* c) {
+  constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
+  if (c->stuned[prec]) return 0;
+  char key[96];
+  snprintf(key, sizeof key, "%d %d %d %d", c->nx, c->ny, c->nz, prec);
+  const char* cache = getenv("APHID_TUNE_CACHE");
+  if (cache) {
+    if (FILE* f = fopen(cache, "r")) {
+      char line[200];
+      while (fgets(line, sizeof line, f)) {
+        int a, b, cc, p, v[9];
+        if (sscanf(line, "S %d %d %d %d %d %d %d %d %d %d %d %d %d", &a, &b, &cc, &p, &v[0], &v[1],
+                   &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8]) != 13)
+          continue;
+        char k2[96];
+        snprintf(k2, sizeof k2, "%d %d %d %d", a, b, cc, p);
+        if (strcmp(k2, key)) continue;
+        bool ok = true;
+        Stream sh[SK_COUNT];
+        for (int q = 0; q < SK_COUNT; ++q) {
+          sh[q] = {v[3 * q], v[3 * q + 1], v[3 * q + 2]};
+          ok = ok && (sh[q].u == 2 || sh[q].u == 4) && sh[q].grid >= 1 &&
+               sh[q].grid <= kMaxPartBlocks && (sh[q].nt == 0 || sh[q].nt == 1);
+        }
+        if (ok) {
+          for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = sh[q];
+          c->stuned[prec] = true;
+        }
+      }
+      fclose(f);
+      if (c->stuned[prec]) return 0;
+    }
+  }
+  int rc = autotune(c, prec);
+  if (rc) return rc;
+  rc = ensure_vectors(c, prec == 1);
+  if (rc) return rc;
+  const long long n4 = 4 * c->nn;
+  Vecs<V> w = vecs<V>(c);
+  w.x = reinterpret_cast<cplx<V>*>(c->V[8]);
+  const cplx<V>* invd = invd_of<V>(c);
+  const Tile& tl = c->tile[prec];
+  QmrState ts{};
+  ts.k1 = {0.5, 0.25};
+  ts.beta = {1.0, 0.5};
+  ts.eta = {0.5, 0.0};
+  ts.eps = {1.0, 0.0};
+  ts.del = {1.0, 0.0};
+  ts.k2s = 0.5;
+  ts.rho_np = ts.rho_n = 1.0;
+  ts.c_nm = 1.0;
+  ts.max_it = 1 << 30;
+  ts.iter = 1;
+  const int reps = c->nn > 2000000 ? 3 : 8;
+  auto launch = [&](int k, const Stream (&sh)[SK_COUNT]) -> hipError_t {
+    switch (k) {
+      case 0:
+        return launch_pupdate<V>(sh[SK_PUPDATE], w.v, invd, w.p, n4, c->tst, c->s);
+      case 1: {
+        ApplyEpilogue ep{c->tst, c->part};
+        const hipError_t e = launch_apply<V>(tl, opargs<V>(c), w.p, w.t, ep, c->s);
+        hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
+                           (int)apply_blocks(tl, c->nx, c->ny, c->nz), c->tst);
+        return e;
+      }
+      case 2:
+        return launch_vtnorm<V>(sh[SK_VTNORM], w.t, w.v, n4, c->tst, c->part, c->s);
+      default:
+        return launch_update<V>(sh[SK_UPDATE], w, invd, n4, c->tst, c->part, nullptr, c->s);
+    }
+  };
+  // mean over reps of the summed per-kernel times of one iteration
+  auto seq_time = [&](const Stream (&sh)[SK_COUNT], float* ms) -> int {
+    float tot = 0.f;
+    for (int r = 0; r <= reps; ++r)  // the first sweep warms up
+      for (int k = 0; k < 4; ++k) {
+        CK(hipMemcpyAsync(c->tst, &ts, sizeof ts, hipMemcpyHostToDevice, c->s));
+        CK(hipEventRecord(c->ev0, c->s));
+        CK(launch(k, sh));
+        CK(hipEventRecord(c->ev1, c->s));
+        CK(hipEventSynchronize(c->ev1));
+        float m = 0.f;
+        CK(hipEventElapsedTime(&m, c->ev0, c->ev1));
+        if (r) tot += m;
+      }
+    *ms = tot / reps;
+    return 0;
+  };
+  Stream cur[SK_COUNT] = {kStreamDefault[0], kStreamDefault[1], kStreamDefault[2]};
+  float cur_ms = 0.f;
+  rc = seq_time(cur, &cur_ms);
+  if (rc) return rc;
+  const float def_ms = cur_ms;
+  const int grids[] = {2048, 4096, 8192, 16384};
+  const long long max_grid = std::max<long long>(2048, (n4 + kBlock - 1) / kBlock);
+  for (int q : {SK_UPDATE, SK_PUPDATE, SK_VTNORM, SK_UPDATE}) {
+    for (int u : {2, 4})
+      for (int g : grids)
+        for (int nt : {0, 1}) {
+          if (g > max_grid) continue;
+          Stream trial[SK_COUNT] = {cur[0], cur[1], cur[2]};
+          trial[q] = {u, g, nt};
+          float ms = 0.f;
+          rc = seq_time(trial, &ms);
+          if (rc) return rc;
+          if (tune_verbose())
+            fprintf(stderr,
+                    "[aphid] stream tune prec %d kernel %d: u %d grid %d nt %d: %.1f us "
+                    "(current %.1f, default %.1f)\n",
+                    prec, q, u, g, nt, 1e3 * ms, 1e3 * cur_ms, 1e3 * def_ms);
+          if (ms < 0.99f * cur_ms) {
+            cur_ms = ms;
+            cur[q] = trial[q];
+          }
+        }
+  }
+  if (cur_ms < 0.97f * def_ms)
+    for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = cur[q];
+  else
+    for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = kStreamDefault[q];
+  c->stuned[prec] = true;
+  if (tune_verbose()) {
+    fprintf(stderr, "[aphid] stream shapes prec %d (%.1f us, default %.1f):", prec, 1e3 * cur_ms,
+            1e3 * def_ms);
+    for (int q = 0; q < SK_COUNT; ++q)
+      fprintf(stderr, " %d %d %d", c->stream[prec][q].u, c->stream[prec][q].grid,
+              c->stream[prec][q].nt);
+    fprintf(stderr, "\n");
+  }
+  if (cache)
+    if (FILE* f = fopen(cache, "a")) {
+      fprintf(f, "S %s", key);
+      for (int q = 0; q < SK_COUNT; ++q)
+        fprintf(f, " %d %d %d", c->stream[prec][q].u, c->stream[prec][q].grid,
+                c->stream[prec][q].nt);
+      fprintf(f, "\n");
+      fclose(f);
+    }
+  return 0;
+}
+
 template <typename V>
 int enqueue_iteration(aphid_ctx* c) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
   const Vecs<V> w = vecs<V>(c);
   const long long n4 = 4 * c->nn;
   const cplx<V>* invd = invd_of<V>(c);
-  hipLaunchKernelGGL((k_pupdate<V, 4>), dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.v, invd, w.p, n4,
-                     c->st);
+  const Stream* sh = c->stream[prec];
+  CK(launch_pupdate<V>(sh[SK_PUPDATE], w.v, invd, w.p, n4, c->st, c->s));
   ApplyEpilogue ep{c->st, c->part};
   const Tile& tl = c->tile[prec];
   CK(launch_apply<V>(tl, opargs<V>(c), w.p, w.t, ep, c->s));
   hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
                      (int)apply_blocks(tl, c->nx, c->ny, c->nz), c->st);
-  hipLaunchKernelGGL(k_vtnorm<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.t, w.v, n4, c->st,
-                     c->part);
-  hipLaunchKernelGGL(k_fin_vtnorm, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st);
-  hipLaunchKernelGGL(k_update<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.p, w.d, w.x, w.t, w.s,
-                     w.r, w.v, invd, n4, c->st, c->part);
-  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st,
-                     c->hist);
+  CK(launch_vtnorm<V>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s));
+  CK(launch_update<V>(sh[SK_UPDATE], w, invd, n4, c->st, c->part, c->hist, c->s));
   CK(hipGetLastError());
   return 0;
 }
@@ -548,6 +771,8 @@ int run_loop(aphid_ctx* c) {
 template <typename V>
 int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
+  int trc = tune_streams<V>(c);
+  if (trc) return trc;
   const Vecs<V> w = vecs<V>(c);
   const long long n4 = 4 * c->nn;
   QmrState init{};
@@ -699,6 +924,7 @@ int aphid_ctx_create(aphid_ctx** out, int device) {
   c->dev = device;
   CK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
   CK(hipMalloc(&c->st, sizeof(QmrState)));
+  CK(hipMalloc(&c->tst, sizeof(QmrState)));
   CK(hipHostMalloc(&c->hst, sizeof(QmrState), hipHostMallocDefault));
   CK(hipMalloc(&c->part, sizeof(double) * 4 * kMaxPartBlocks));
   CK(hipMalloc(&c->resbuf, sizeof(double) * 4));
@@ -715,6 +941,7 @@ int aphid_ctx_destroy(aphid_ctx* c) {
   if (c->s) hipStreamSynchronize(c->s);
   free_mesh(c);
   if (c->st) hipFree(c->st);
+  if (c->tst) hipFree(c->tst);
   if (c->hst) hipHostFree(c->hst);
   if (c->part) hipFree(c->part);
   if (c->resbuf) hipFree(c->resbuf);
@@ -1447,35 +1674,31 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   CK(hipMemcpyAsync(c->st, &hs, sizeof hs, hipMemcpyHostToDevice, c->s));
   const Vecs<double> w = vecs<double>(c);
   const long long n4 = 4 * c->nn;
+  const Stream* sh = c->stream[0];
   auto K0 = [&]() {
-    hipLaunchKernelGGL((k_pupdate<double, 4>), dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.v,
-                       c->invd, w.p, n4, c->st);
+    return launch_pupdate<double>(sh[SK_PUPDATE], w.v, c->invd, w.p, n4, c->st, c->s);
   };
   auto K1 = [&]() {
     ApplyEpilogue ep{c->st, c->part};
-    launch_apply<double>(c->tile[0], c->op64, w.p, w.t, ep, c->s);
+    const hipError_t e = launch_apply<double>(c->tile[0], c->op64, w.p, w.t, ep, c->s);
     hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
                        (int)apply_blocks(c->tile[0], c->nx, c->ny, c->nz), c->st);
+    return e;
   };
   auto K2 = [&]() {
-    hipLaunchKernelGGL(k_vtnorm<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.t, w.v, n4,
-                       c->st, c->part);
-    hipLaunchKernelGGL(k_fin_vtnorm, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st);
+    return launch_vtnorm<double>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s);
   };
   auto K3 = [&]() {
-    hipLaunchKernelGGL(k_update<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.p, w.d, w.x,
-                       w.t, w.s, w.r, w.v, c->invd, n4, c->st, c->part);
-    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, c->s, c->part, kRedBlocks, c->st,
-                       nullptr);
+    return launch_update<double>(sh[SK_UPDATE], w, c->invd, n4, c->st, c->part, nullptr, c->s);
   };
   float ms[5] = {0, 0, 0, 0, 0};
   for (int rep = 0; rep < reps + 1; ++rep) {  // first sweep warms up
     for (int q = 0; q < 4; ++q) {
       CK(hipEventRecord(c->ev0, c->s));
-      if (q == 0) K0();
-      if (q == 1) K1();
-      if (q == 2) K2();
-      if (q == 3) K3();
+      if (q == 0) CK(K0());
+      if (q == 1) CK(K1());
+      if (q == 2) CK(K2());
+      if (q == 3) CK(K3());
       CK(hipEventRecord(c->ev1, c->s));
       CK(hipEventSynchronize(c->ev1));
       float t = 0.f;
@@ -1486,10 +1709,10 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   CK(hipGetLastError());
   CK(hipEventRecord(c->ev0, c->s));
   for (int rep = 0; rep < reps; ++rep) {
-    K0();
-    K1();
-    K2();
-    K3();
+    CK(K0());
+    CK(K1());
+    CK(K2());
+    CK(K3());
   }
   CK(hipEventRecord(c->ev1, c->s));
   CK(hipEventSynchronize(c->ev1));

@@ -121,14 +121,38 @@ __global__ void k_fin_init_v(const double* part, int nb, QmrState* st) {
 // Streaming kernels below unroll U elements per thread and step (loads of
 // all U first, then arithmetic, then stores) so each wave keeps U times
 // more bytes in flight; element order inside a thread is fixed, so the
-// reductions stay deterministic.
+// reductions stay deterministic for a given launch shape.
 #define UNROLLED_LOOP(U, q, stride, n)                                       \
   const long long stride = (long long)gridDim.x * blockDim.x;                 \
   long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;             \
   for (; q + (U - 1) * stride < (n); q += (U) * stride)
 
+// Vector element loads and stores; NT marks them non-temporal (the `nt`
+// bit of global_load/store_dwordx4), a streaming hint: an iteration
+// touches 13 Krylov vectors once each, far more than the L2 and the
+// Infinity Cache hold.  Which hint and launch shape is fastest depends on
+// the vector length (HBM channel interleaving), so the solver times the
+// candidates per mesh (stream shapes in aphid_gpu.hip).
+template <bool NT, typename T>
+__device__ __forceinline__ cplx<T> ldv(const cplx<T>* p) {
+  if constexpr (NT) {
+    return {__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+  } else {
+    return *p;
+  }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void stv(cplx<T>* p, cplx<T> v) {
+  if constexpr (NT) {
+    __builtin_nontemporal_store(v.re, &p->re);
+    __builtin_nontemporal_store(v.im, &p->im);
+  } else {
+    *p = v;
+  }
+}
+
 // A: p = M^-1 v - p k1 (:306-320)
-template <typename V, int U = 4>
+template <typename V, int U = 4, bool NT = false>
 __global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
                           vec<V>* __restrict__ p, long long n, const QmrState* st) {
   if (st->done) return;
@@ -137,34 +161,37 @@ __global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict
     vec<V> a[U], b[U], c[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      a[u] = v[q + u * stride];
-      b[u] = invd[q + u * stride];
-      c[u] = p[q + u * stride];
+      a[u] = ldv<NT>(v + q + u * stride);
+      b[u] = ldv<NT>(invd + q + u * stride);
+      c[u] = ldv<NT>(p + q + u * stride);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) p[q + u * stride] = dn<V>(up(a[u]) * up(b[u]) - up(c[u]) * k1);
+    for (int u = 0; u < U; ++u)
+      stv<NT>(p + q + u * stride, dn<V>(up(a[u]) * up(b[u]) - up(c[u]) * k1));
   }
-  for (; q < n; q += stride) p[q] = dn<V>(up(v[q]) * up(invd[q]) - up(p[q]) * k1);
+  for (; q < n; q += stride)
+    stv<NT>(p + q, dn<V>(up(ldv<NT>(v + q)) * up(ldv<NT>(invd + q)) - up(ldv<NT>(p + q)) * k1));
 }
 
 // C: rho' = ||t - beta v|| and the step-4 scalars (:332-345)
-template <typename V>
+template <typename V, int U = 4, bool NT = false>
 __global__ void k_vtnorm(const vec<V>* __restrict__ t, const vec<V>* __restrict__ v, long long n,
                          const QmrState* st, double* part) {
   if (st->done) return;
   const c128 beta = st->beta;
   double a[1] = {0.0};
-  UNROLLED_LOOP(4, q, stride, n) {
-    vec<V> tt[4], vv[4];
+  UNROLLED_LOOP(U, q, stride, n) {
+    vec<V> tt[U], vv[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      tt[u] = t[q + u * stride];
-      vv[u] = v[q + u * stride];
+    for (int u = 0; u < U; ++u) {
+      tt[u] = ldv<NT>(t + q + u * stride);
+      vv[u] = ldv<NT>(v + q + u * stride);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[0] += abs2(up(dn<V>(up(tt[u]) - up(vv[u]) * beta)));
+    for (int u = 0; u < U; ++u) a[0] += abs2(up(dn<V>(up(tt[u]) - up(vv[u]) * beta)));
   }
-  for (; q < n; q += stride) a[0] += abs2(up(dn<V>(up(t[q]) - up(v[q]) * beta)));
+  for (; q < n; q += stride)
+    a[0] += abs2(up(dn<V>(up(ldv<NT>(t + q)) - up(ldv<NT>(v + q)) * beta)));
   block_sum<1>(a);
   block_partial<1>(a, part);
 }
@@ -193,7 +220,7 @@ __global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
 
 // one element of step D: d, x, s, r, v updates; accumulates ||r||^2 and
 // v^T M^-1 v (src/aphid_qmr.f90:347-369, 306-311)
-template <typename V>
+template <typename V, bool NT = false>
 __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_,
                                             vec<V> sq, vec<V> rq, vec<V> vq, vec<V> iq, c128 eta,
                                             c128 beta, double k2, double rho,
@@ -202,15 +229,15 @@ __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, v
                                             vec<V>* __restrict__ v, double (&a)[3]) {
   const c128 tq = up(tq_);
   const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
-  d[q] = ddv;
-  x[q] = dn<V>(up(xq) + up(ddv));
+  stv<NT>(d + q, ddv);
+  stv<NT>(x + q, dn<V>(up(xq) + up(ddv)));
   const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
-  s[q] = ssv;
+  stv<NT>(s + q, ssv);
   const vec<V> rrv = dn<V>(up(rq) - up(ssv));
-  r[q] = rrv;
+  stv<NT>(r + q, rrv);
   const vec<V> vt = dn<V>(tq - up(vq) * beta);
   const vec<V> vv = dn<V>(divr(up(vt), rho));
-  v[q] = vv;
+  stv<NT>(v + q, vv);
   a[0] += abs2(up(rrv));
   const c128 vd = up(vv);
   const c128 pr = vd * (vd * up(iq));
@@ -221,7 +248,7 @@ __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, v
 // D: d = eta p + k2 d; x += d; s = eta t + k2 s; r -= s; v = (t - beta v)/rho';
 // ||r|| and the stop test (:347-390), then del = v^T M^-1 v and k1 for the
 // next iteration (:306-313).
-template <typename V>
+template <typename V, int U = 2, bool NT = false>
 __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x,
                          const vec<V>* __restrict__ t, vec<V>* __restrict__ s, vec<V>* __restrict__ r,
                          vec<V>* __restrict__ v, const vec<V>* __restrict__ invd, long long n,
@@ -232,20 +259,23 @@ __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, v
   double a[3] = {0.0, 0.0, 0.0};
   auto body = [&](long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_, vec<V> sq, vec<V> rq,
                   vec<V> vq, vec<V> iq) {
-    update_elem<V>(q, pq, dq, xq, tq_, sq, rq, vq, iq, eta, beta, k2, rho, d, x, s, r, v, a);
+    update_elem<V, NT>(q, pq, dq, xq, tq_, sq, rq, vq, iq, eta, beta, k2, rho, d, x, s, r, v, a);
   };
-  UNROLLED_LOOP(2, q, stride, n) {
-    vec<V> P_[2], D_[2], X_[2], T_[2], S_[2], R_[2], Vv[2], I_[2];
+  UNROLLED_LOOP(U, q, stride, n) {
+    vec<V> P_[U], D_[U], X_[U], T_[U], S_[U], R_[U], Vv[U], I_[U];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
       const long long e = q + u * stride;
-      P_[u] = p[e]; D_[u] = d[e]; X_[u] = x[e]; T_[u] = t[e];
-      S_[u] = s[e]; R_[u] = r[e]; Vv[u] = v[e]; I_[u] = invd[e];
+      P_[u] = ldv<NT>(p + e); D_[u] = ldv<NT>(d + e); X_[u] = ldv<NT>(x + e);
+      T_[u] = ldv<NT>(t + e); S_[u] = ldv<NT>(s + e); R_[u] = ldv<NT>(r + e);
+      Vv[u] = ldv<NT>(v + e); I_[u] = ldv<NT>(invd + e);
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) body(q + u * stride, P_[u], D_[u], X_[u], T_[u], S_[u], R_[u], Vv[u], I_[u]);
+    for (int u = 0; u < U; ++u) body(q + u * stride, P_[u], D_[u], X_[u], T_[u], S_[u], R_[u], Vv[u], I_[u]);
   }
-  for (; q < n; q += stride) body(q, p[q], d[q], x[q], t[q], s[q], r[q], v[q], invd[q]);
+  for (; q < n; q += stride)
+    body(q, ldv<NT>(p + q), ldv<NT>(d + q), ldv<NT>(x + q), ldv<NT>(t + q), ldv<NT>(s + q),
+         ldv<NT>(r + q), ldv<NT>(v + q), ldv<NT>(invd + q));
   block_sum<3>(a);
   block_partial<3>(a, part);
 }
