@@ -9,7 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "aphid_gpu.h"
@@ -170,6 +173,7 @@ struct aphid_ctx {
   bool have_k2 = false, have_diag = false;
   // Krylov vectors, complex128 and complex64 sets
   c128* V[9] = {};  // x b v p t d s r spare
+  bool vborrowed = false;  // V[] are slices of a batch's flat vectors (not owned)
   c64* F[8] = {};   // x b v p t d s r (complex64)
   c128* stage = nullptr;
   float* stagef = nullptr;  // staging for FP32 cell inputs (4 blocks)
@@ -183,6 +187,7 @@ struct aphid_ctx {
   // graph of `giters` iterations per precision
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int giters[2] = {0, 0};
+  const void* gvec[2] = {nullptr, nullptr};  // x vector the graph was captured with
   // stencil tiles per precision
   Tile tile[2] = {{16, 16, 8}, {16, 16, 8}};
   bool tuned[2] = {false, false};
@@ -212,8 +217,11 @@ void free_mesh(aphid_ctx* c) {
                   c->stage, c->stagef, c->hist};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  for (auto& v : c->V)
-    if (v) hipFree(v), v = nullptr;
+  for (auto& v : c->V) {
+    if (v && !c->vborrowed) hipFree(v);
+    v = nullptr;
+  }
+  c->vborrowed = false;
   for (auto& v : c->F)
     if (v) hipFree(v), v = nullptr;
   c->tab64 = nullptr; c->tab32 = nullptr; c->dmet = nullptr; c->sig = nullptr; c->k2 = nullptr;
@@ -324,11 +332,37 @@ int time_tile(aphid_ctx* c, const Tile& t, const OpArgs<R>& op, const cplx<R>* i
   return 0;
 }
 
+// Process-wide memo of tuned launch parameters per (device, mesh shape,
+// precision): contexts and batch operators on a shape already tuned in
+// this process skip the timing.
+struct TuneKey {
+  int dev, nx, ny, nz, prec;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dev, nx, ny, nz, prec) < std::tie(o.dev, o.nx, o.ny, o.nz, o.prec);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Tile> g_tiles;
+struct StreamSet {
+  Stream sh[SK_COUNT];
+};
+std::map<TuneKey, StreamSet> g_streams;
+
 // Autotuner: time every (tile, chunk) candidate for this mesh and keep the
 // fastest.  Results are cached per mesh shape and precision in the file
-// named by APHID_TUNE_CACHE (if set).
+// named by APHID_TUNE_CACHE (if set), and in memory for the process.
 int autotune(aphid_ctx* c, int prec) {
   if (c->tuned[prec]) return 0;
+  const TuneKey tk0{c->dev, c->nx, c->ny, c->nz, prec};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tiles.find(tk0);
+    if (it != g_tiles.end()) {
+      c->tile[prec] = it->second;
+      c->tuned[prec] = true;
+      return 0;
+    }
+  }
   char key[96];
   snprintf(key, sizeof key, "%d %d %d %d", c->nx, c->ny, c->nz, prec);
   const char* cache = getenv("APHID_TUNE_CACHE");
@@ -347,7 +381,11 @@ int autotune(aphid_ctx* c, int prec) {
         }
       }
       fclose(f);
-      if (c->tuned[prec]) return 0;
+      if (c->tuned[prec]) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tiles[tk0] = c->tile[prec];
+        return 0;
+      }
     }
   }
   int rc = ensure_vectors(c, prec == 1);
@@ -374,6 +412,10 @@ int autotune(aphid_ctx* c, int prec) {
   }
   c->tile[prec] = bt;
   c->tuned[prec] = true;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tiles[tk0] = bt;
+  }
   if (cache)
     if (FILE* f = fopen(cache, "a")) {
       fprintf(f, "%s %d %d %d\n", key, bt.tk, bt.tj, bt.chunk);
@@ -478,7 +520,8 @@ bool tune_verbose() {
 // stores changes how fast the stencil then reads p.  So the tuner times the
 // iteration's own kernel sequence (p-update, stencil with eps, ||t - beta
 // v||, fused update), each kernel with its finalize, and picks the shapes
-// by coordinate descent on the sequence total.  The tuned set replaces the
+// by one coordinate-descent pass on the sequence total (fused update, then
+// p-update, then ||t - beta v||).  The tuned set replaces the
 // defaults only when the total is at least 3% lower, so that near-ties do
 // not flip the launch shapes (and with them the order of the reductions)
 // from run to run.  Timing runs on vectors that the solve overwrites
@@ -491,6 +534,16 @@ template <typename V>
 int tune_streams(aphid_ctx* c) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
   if (c->stuned[prec]) return 0;
+  const TuneKey tk0{c->dev, c->nx, c->ny, c->nz, prec};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_streams.find(tk0);
+    if (it != g_streams.end()) {
+      for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = it->second.sh[q];
+      c->stuned[prec] = true;
+      return 0;
+    }
+  }
   char key[96];
   snprintf(key, sizeof key, "%d %d %d %d", c->nx, c->ny, c->nz, prec);
   const char* cache = getenv("APHID_TUNE_CACHE");
@@ -518,7 +571,11 @@ int tune_streams(aphid_ctx* c) {
         }
       }
       fclose(f);
-      if (c->stuned[prec]) return 0;
+      if (c->stuned[prec]) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (int q = 0; q < SK_COUNT; ++q) g_streams[tk0].sh[q] = c->stream[prec][q];
+        return 0;
+      }
     }
   }
   int rc = autotune(c, prec);
@@ -583,7 +640,7 @@ int tune_streams(aphid_ctx* c) {
   const float def_ms = cur_ms;
   const int grids[] = {2048, 4096, 8192, 16384};
   const long long max_grid = std::max<long long>(2048, (n4 + kBlock - 1) / kBlock);
-  for (int q : {SK_UPDATE, SK_PUPDATE, SK_VTNORM, SK_UPDATE}) {
+  for (int q : {SK_UPDATE, SK_PUPDATE, SK_VTNORM}) {
     for (int u : {2, 4})
       for (int g : grids)
         for (int nt : {0, 1}) {
@@ -609,6 +666,10 @@ int tune_streams(aphid_ctx* c) {
   else
     for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = kStreamDefault[q];
   c->stuned[prec] = true;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int q = 0; q < SK_COUNT; ++q) g_streams[tk0].sh[q] = c->stream[prec][q];
+  }
   if (tune_verbose()) {
     fprintf(stderr, "[aphid] stream shapes prec %d (%.1f us, default %.1f):", prec, 1e3 * cur_ms,
             1e3 * def_ms);
@@ -730,6 +791,11 @@ int run_loop(aphid_ctx* c) {
       last_iter = c->hst->iter;
     }
   }
+  const void* xv = vecs<V>(c).x;
+  if (c->gexec[prec] && c->gvec[prec] != xv) {  // vectors moved since the capture
+    CK(hipGraphExecDestroy(c->gexec[prec]));
+    c->gexec[prec] = nullptr;
+  }
   if (!c->gexec[prec]) {
     const int k = graph_iters(c, prec);
     hipGraph_t g;
@@ -742,6 +808,7 @@ int run_loop(aphid_ctx* c) {
     CK(hipGraphInstantiate(&c->gexec[prec], g, nullptr, nullptr, 0));
     CK(hipGraphDestroy(g));
     c->giters[prec] = k;
+    c->gvec[prec] = xv;
   }
   // every replay advances the iteration counter by giters unless the solve
   // stops; bound the replays so a state that fails to advance is an error,
@@ -1803,6 +1870,12 @@ struct aphid_batch {
   c128* stage = nullptr;
   long long stage_n = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // non-temporal hints of the streaming kernels per precision (p-update,
+  // ||t - beta v||, fused update), chosen by timing; scratch states for it
+  int nts[2][SK_COUNT] = {};
+  bool ntuned[2] = {false, false};
+  QmrState* tst = nullptr;
+  int* dskip = nullptr;  // [nsys] systems the batched launches leave alone
 };
 
 namespace {
@@ -1822,12 +1895,13 @@ void batch_free_device(aphid_batch* B) {
   for (auto& v : B->V)
     if (v) hipFree(v), v = nullptr;
   void* ps[] = {B->invd, B->dops, B->doff, B->dn4, B->dopoff, B->dop, B->dbstart, B->dtstart,
-                B->st, B->part, B->res};
+                B->st, B->part, B->res, B->tst, B->dskip};
   for (void* p : ps)
     if (p) hipFree(p);
   B->invd = nullptr; B->dops = nullptr; B->doff = nullptr; B->dn4 = nullptr; B->dopoff = nullptr;
   B->dop = nullptr; B->dbstart = nullptr; B->dtstart = nullptr; B->st = nullptr; B->part = nullptr;
-  B->res = nullptr;
+  B->res = nullptr; B->tst = nullptr; B->dskip = nullptr;
+  B->ntuned[0] = B->ntuned[1] = false;
   B->built = false;
 }
 
@@ -1901,6 +1975,45 @@ int batch_build(aphid_batch* B) {
   return 0;
 }
 
+// kernel k (0 p-update, 1 stencil with eps, 2 ||t - beta v||, 3 fused
+// update) of a batch iteration, each with its finalize, with the
+// non-temporal hints nts
+template <typename V>
+void batch_launch(aphid_batch* B, const BatchDev& D, int k, const int* nts,
+                  const OpArgs<V>* ops, cplx<V>* x, cplx<V>* v, cplx<V>* p, cplx<V>* t, cplx<V>* d,
+                  cplx<V>* s, cplx<V>* r, const cplx<V>* invd) {
+  const int ns = (int)B->sys.size();
+  const dim3 gs(B->nstream), bs(kBlock);
+  switch (k) {
+    case 0:
+      if (nts[SK_PUPDATE])
+        hipLaunchKernelGGL((kb_pupdate<V, true>), gs, bs, 0, B->s, D, v, invd, p);
+      else
+        hipLaunchKernelGGL((kb_pupdate<V, false>), gs, bs, 0, B->s, D, v, invd, p);
+      break;
+    case 1:
+      hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, p, t, 1,
+                         B->part);
+      hipLaunchKernelGGL(kb_fin_eps, dim3(ns), bs, 0, B->s, D, B->part);
+      break;
+    case 2:
+      if (nts[SK_VTNORM])
+        hipLaunchKernelGGL((kb_vtnorm<V, true>), gs, bs, 0, B->s, D, t, v, B->part);
+      else
+        hipLaunchKernelGGL((kb_vtnorm<V, false>), gs, bs, 0, B->s, D, t, v, B->part);
+      hipLaunchKernelGGL(kb_fin_vtnorm, dim3(ns), bs, 0, B->s, D, B->part);
+      break;
+    default:
+      if (nts[SK_UPDATE])
+        hipLaunchKernelGGL((kb_update<V, true>), gs, bs, 0, B->s, D, p, d, x, t, s, r, v, invd,
+                           B->part);
+      else
+        hipLaunchKernelGGL((kb_update<V, false>), gs, bs, 0, B->s, D, p, d, x, t, s, r, v, invd,
+                           B->part);
+      hipLaunchKernelGGL(kb_fin_update, dim3(ns), bs, 0, B->s, D, B->part);
+  }
+}
+
 template <typename V>
 int batch_enqueue_iteration(aphid_batch* B) {
   const BatchDev D = batch_dev(B);
@@ -1917,16 +2030,86 @@ int batch_enqueue_iteration(aphid_batch* B) {
   const OpArgs<V>* ops;
   if constexpr (std::is_same<V, double>::value) ops = B->dops;
   else ops = B->dops32;
-  hipLaunchKernelGGL(kb_pupdate<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, v, invd, p);
-  hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, p, t, 1,
-                     B->part);
-  hipLaunchKernelGGL(kb_fin_eps, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
-  hipLaunchKernelGGL(kb_vtnorm<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, t, v, B->part);
-  hipLaunchKernelGGL(kb_fin_vtnorm, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
-  hipLaunchKernelGGL(kb_update<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, p, d, x, t, s, r, v,
-                     invd, B->part);
-  hipLaunchKernelGGL(kb_fin_update, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
+  const int* nts = B->nts[std::is_same<V, double>::value ? 0 : 1];
+  for (int k = 0; k < 4; ++k)
+    batch_launch<V>(B, D, k, nts, ops, x, v, p, t, d, s, r, invd);
   CK(hipGetLastError());
+  return 0;
+}
+
+// Non-temporal hints of the batch's streaming kernels: all 8 combinations
+// timed over the iteration's kernel sequence (each kernel with its
+// finalize, scratch states reloaded before every launch, on vectors the
+// solve overwrites before it reads them; the spare stands in for x).  A
+// combination replaces plain loads and stores only when the sequence is at
+// least 3% faster.
+template <typename V>
+int batch_tune_nt(aphid_batch* B) {
+  constexpr int prec = std::is_same<V, double>::value ? 0 : 1;
+  if (B->ntuned[prec]) return 0;
+  const int ns = (int)B->sys.size();
+  if (!B->tst) CK(hipMalloc(&B->tst, sizeof(QmrState) * ns));
+  QmrState t1{};
+  t1.k1 = {0.5, 0.25};
+  t1.beta = {1.0, 0.5};
+  t1.eta = {0.5, 0.0};
+  t1.eps = {1.0, 0.0};
+  t1.del = {1.0, 0.0};
+  t1.k2s = 0.5;
+  t1.rho_np = t1.rho_n = 1.0;
+  t1.c_nm = 1.0;
+  t1.max_it = 1 << 30;
+  t1.iter = 1;
+  const std::vector<QmrState> ts(ns, t1);
+  BatchDev D = batch_dev(B);
+  D.st = B->tst;
+  cplx<V>*x, *v, *p, *t, *d, *s, *r;
+  const cplx<V>* invd;
+  const OpArgs<V>* ops;
+  x = reinterpret_cast<cplx<V>*>(B->V[8]);
+  if constexpr (prec == 0) {
+    v = B->V[2]; p = B->V[3]; t = B->V[4]; d = B->V[5]; s = B->V[6]; r = B->V[7];
+    invd = B->invd; ops = B->dops;
+  } else {
+    v = B->F[2]; p = B->F[3]; t = B->F[4]; d = B->F[5]; s = B->F[6]; r = B->F[7];
+    invd = B->invdf; ops = B->dops32;
+  }
+  const int reps = B->n4_tot > 8000000 ? 3 : 8;
+  auto seq_time = [&](const int (&nt)[SK_COUNT], float* ms) -> int {
+    float tot = 0.f;
+    for (int rep = 0; rep <= reps; ++rep)
+      for (int k = 0; k < 4; ++k) {
+        CK(hipMemcpyAsync(B->tst, ts.data(), sizeof(QmrState) * ns, hipMemcpyHostToDevice, B->s));
+        CK(hipEventRecord(B->ev0, B->s));
+        batch_launch<V>(B, D, k, nt, ops, x, v, p, t, d, s, r, invd);
+        CK(hipGetLastError());
+        CK(hipEventRecord(B->ev1, B->s));
+        CK(hipEventSynchronize(B->ev1));
+        float m = 0.f;
+        CK(hipEventElapsedTime(&m, B->ev0, B->ev1));
+        if (rep) tot += m;
+      }
+    *ms = tot / reps;
+    return 0;
+  };
+  float def_ms = 0.f, best_ms = 0.f;
+  int best[SK_COUNT] = {0, 0, 0};
+  for (int m = 0; m < 8; ++m) {
+    const int nt[SK_COUNT] = {m & 1, (m >> 1) & 1, (m >> 2) & 1};
+    float ms = 0.f;
+    const int rc = seq_time(nt, &ms);
+    if (rc) return rc;
+    if (tune_verbose())
+      fprintf(stderr, "[aphid] batch nt prec %d: %d %d %d: %.1f us\n", prec, nt[0], nt[1], nt[2],
+              1e3 * ms);
+    if (m == 0) def_ms = best_ms = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      for (int q = 0; q < SK_COUNT; ++q) best[q] = nt[q];
+    }
+  }
+  for (int q = 0; q < SK_COUNT; ++q) B->nts[prec][q] = best_ms < 0.97f * def_ms ? best[q] : 0;
+  B->ntuned[prec] = true;
   return 0;
 }
 
@@ -2063,8 +2246,10 @@ int batch_build_f32(aphid_batch* B) {
 // of that precision; per-system stop targets itol[s]*||b_s|| (or tol).
 template <typename V>
 int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool use_x0,
-              const int* maxits = nullptr) {
+              const int* maxits = nullptr, const int* skip = nullptr) {
   constexpr bool f64 = std::is_same<V, double>::value;
+  int trc = batch_tune_nt<V>(B);
+  if (trc) return trc;
   const BatchDev D = batch_dev(B);
   const int ns = (int)B->sys.size();
   cplx<V>*x, *b, *v, *p, *t, *d, *s_, *r;
@@ -2092,7 +2277,7 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
   hipLaunchKernelGGL(kb_init_v<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, r, invd, v, p, d,
                      s_, B->part);
   hipLaunchKernelGGL(kb_fin_init_v, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, max_it, 0,
-                     maxits);
+                     maxits, skip);
   CK(hipGetLastError());
   int rc = batch_fetch_states(B);
   if (rc) return rc;
@@ -2128,6 +2313,36 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
     if (batch_all_done(B)) return 0;
     if (l > max_launch) return fail(APHID_E_STATE, "batched QMR state did not advance");
   }
+}
+
+// Systems of at least this many unknowns (4 nn) take the single-system
+// path inside a batch solve: one such system alone fills the GPU's
+// bandwidth, and the single-system kernels (tuned stencil tiles and stream
+// shapes, and a working set that may stay in the Infinity Cache) beat the
+// batched ones.  APHID_BATCH_SEQ_MIN overrides the threshold.
+long long batch_seq_min() {
+  const char* e = getenv("APHID_BATCH_SEQ_MIN");
+  const long long v = e ? atoll(e) : 0;
+  return v > 0 ? v : (1LL << 20);
+}
+
+// Solve system q on its operator's context, in place on the batch's
+// vectors: the context borrows the system's slices of the flat arrays.
+int batch_solve_single(aphid_batch* B, int q, double tol, int max_it, bool use_x0,
+                       QmrState* out) {
+  const auto& sy = B->sys[q];
+  aphid_ctx* c = B->ops[sy.op];
+  if (!c->vborrowed)
+    for (auto& v : c->V)
+      if (v) hipFree(v), v = nullptr;
+  for (int i = 0; i < 9; ++i) c->V[i] = B->V[i] + sy.off;
+  c->vborrowed = true;
+  int rc = autotune(c, 0);
+  double secs = 0.0;
+  if (!rc) rc = qmr_device<double>(c, tol, max_it, use_x0, &secs);
+  if (rc) return rc;
+  *out = *c->hst;
+  return 0;
 }
 
 // per-system ||b - K x|| and ||b|| (complex128) into B->res
@@ -2181,10 +2396,33 @@ int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
   std::vector<QmrState> last(ns);
   int rc = 0;
   if (!(flags & APHID_F_MIXED)) {
-    rc = batch_qmr<double>(B, tol, nullptr, max_it, use_x0, nullptr);
-    if (rc) return rc;
-    last = B->hst;
-    for (int q = 0; q < ns; ++q) iters[q] = last[q].iter;
+    // large systems one at a time on the single-system path, the rest in
+    // the batched launches with the large ones masked out
+    const long long seq_min = batch_seq_min();
+    std::vector<int> skip(ns, 0);
+    int nseq = 0;
+    for (int q = 0; q < ns; ++q)
+      if (B->sys[q].n4 >= seq_min) skip[q] = 1, ++nseq;
+    if (nseq) {
+      CK(hipStreamSynchronize(B->s));  // b, x0 and the zeroed x are in place
+      for (int q = 0; q < ns; ++q)
+        if (skip[q]) {
+          rc = batch_solve_single(B, q, tol, max_it, use_x0, &last[q]);
+          if (rc) return rc;
+          iters[q] = last[q].iter;
+        }
+    }
+    if (nseq < ns) {
+      if (!B->dskip) CK(hipMalloc(&B->dskip, sizeof(int) * ns));
+      CK(hipMemcpyAsync(B->dskip, skip.data(), sizeof(int) * ns, hipMemcpyHostToDevice, B->s));
+      rc = batch_qmr<double>(B, tol, nullptr, max_it, use_x0, nullptr, B->dskip);
+      if (rc) return rc;
+      for (int q = 0; q < ns; ++q)
+        if (!skip[q]) {
+          last[q] = B->hst[q];
+          iters[q] = last[q].iter;
+        }
+    }
   } else {
     rc = batch_build_f32(B);
     if (rc) return rc;

@@ -130,13 +130,20 @@ __global__ void kb_init_v(BatchDev B, const vec<V>* __restrict__ r, const vec<V>
   block_sum<2>(a);
   block_partial<2>(a, part);
 }
+// skip (optional): systems solved elsewhere; they start done and untouched
 __global__ void kb_fin_init_v(BatchDev B, const double* part, int max_it, int hist_cap,
-                              const int* __restrict__ maxits) {
+                              const int* __restrict__ maxits, const int* __restrict__ skip) {
   const int s = blockIdx.x;
   double tot[2];
   b_fin_sum<2>(part, B.bstart[s], B.bstart[s + 1], tot);
   if (threadIdx.x == 0) {
     QmrState* st = &B.st[s];
+    if (skip && skip[s]) {
+      st->done = 1;
+      st->status = ST_CONV;
+      st->iter = 0;
+      return;
+    }
     const double rho = st->rho_np;
     st->max_it = maxits ? maxits[s] : max_it;
     st->hist_cap = hist_cap;
@@ -164,7 +171,9 @@ __global__ void kb_fin_init_v(BatchDev B, const double* part, int max_it, int hi
   }
 }
 
-template <typename V>
+// The streaming kernels take the non-temporal hint NTS of qmr.hpp's
+// ldv/stv as a template flag; the batch picks it per kernel by timing.
+template <typename V, bool NTS = false>
 __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
                            vec<V>* __restrict__ p) {
   const BItem it = b_item(B);
@@ -172,7 +181,8 @@ __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V
   if (st->done) return;
   const c128 k1 = st->k1;
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
-  B_LOOP(q, it) p[q] = dn<V>(up(v[q]) * up(invd[q + io]) - up(p[q]) * k1);
+  B_LOOP(q, it)
+    stv<NTS>(p + q, dn<V>(up(ldv<NTS>(v + q)) * up(ldv<NTS>(invd + q + io)) - up(ldv<NTS>(p + q)) * k1));
 }
 
 // batched stencil: one node per thread, work list of (system, x-plane,
@@ -240,7 +250,7 @@ __global__ void kb_fin_eps(BatchDev B, const double* part) {
   }
 }
 
-template <typename V>
+template <typename V, bool NTS = false>
 __global__ void kb_vtnorm(BatchDev B, const vec<V>* __restrict__ t, const vec<V>* __restrict__ v,
                           double* part) {
   const BItem it = b_item(B);
@@ -248,7 +258,7 @@ __global__ void kb_vtnorm(BatchDev B, const vec<V>* __restrict__ t, const vec<V>
   if (st->done) return;
   const c128 beta = st->beta;
   double a[1] = {0.0};
-  B_LOOP(q, it) a[0] += abs2(up(dn<V>(up(t[q]) - up(v[q]) * beta)));
+  B_LOOP(q, it) a[0] += abs2(up(dn<V>(up(ldv<NTS>(t + q)) - up(ldv<NTS>(v + q)) * beta)));
   block_sum<1>(a);
   block_partial<1>(a, part);
 }
@@ -262,7 +272,7 @@ __global__ void kb_fin_vtnorm(BatchDev B, const double* part) {
   if (threadIdx.x == 0) vt_scalars(st, sqrt(tot[0]));
 }
 
-template <typename V>
+template <typename V, bool NTS = false>
 __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __restrict__ d,
                           vec<V>* __restrict__ x, const vec<V>* __restrict__ t, vec<V>* __restrict__ s_,
                           vec<V>* __restrict__ r, vec<V>* __restrict__ v,
@@ -275,8 +285,9 @@ __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __re
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
   double a[3] = {0.0, 0.0, 0.0};
   B_LOOP(q, it)
-    update_elem<V>(q, p[q], d[q], x[q], t[q], s_[q], r[q], v[q], invd[q + io], eta, beta, k2, rho,
-                   d, x, s_, r, v, a);
+    update_elem<V, NTS>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q), ldv<NTS>(t + q),
+                        ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
+                        ldv<NTS>(invd + q + io), eta, beta, k2, rho, d, x, s_, r, v, a);
   block_sum<3>(a);
   block_partial<3>(a, part);
 }

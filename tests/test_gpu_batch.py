@@ -137,3 +137,58 @@ def test_batched_mixed_precision_meets_fp64_accuracy(gpu_mod):
         emix = np.linalg.norm(sols["mixed"][q] - ref) / np.linalg.norm(ref)
         print(f"  system {q}: field error vs tight FP64 solve: c128 {e128:.1e}  mixed {emix:.1e}")
         assert emix <= max(1e-6, 3.0 * e128)
+
+
+def test_large_systems_take_the_single_path_in_place(gpu_mod, monkeypatch):
+    """Systems at or above APHID_BATCH_SEQ_MIN unknowns are solved one at a
+    time on the single-system path, in place on the batch's vectors; the
+    rest share the batched launches.  Two large systems share each operator
+    (the captured iteration graph follows the system's vectors).  Large
+    systems reproduce Solver.qmr_solve exactly (same kernels and launch
+    shapes); small ones match it as the batched path does; a warm restart
+    from the batch's own solutions re-converges within a few iterations on
+    both paths."""
+    base = meshgen.config1()                      # 33^3 nodes: 4 nn = 143,748
+    monkeypatch.setenv("APHID_BATCH_SEQ_MIN", "100000")
+    small = _uneven_cases()[:2]                   # 4 nn < 100,000: batched
+    freqs = [3.0, 300.0]
+    srcs = [(1, 17, 17, 9), (3, 12, 20, 9)]
+    single = gpu_mod.Solver()
+    with gpu_mod.BatchSolver() as B:
+        plan = []                                 # (case-like, omega, model, cells, b)
+        ops_big = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells)
+                   for f in freqs]
+        ops_small = [B.add_case_operator(c) for c, _ in small]
+        single.set_mesh(base.xn, base.yn, base.zn)
+        for o, f in zip(ops_big, freqs):
+            for src in srcs:
+                b, _ = single.build_rhs(5, [src])
+                B.add_system(o, b)
+                plan.append((base, 2 * np.pi * f, "R", base.cells, b, True))
+        for o, (c, ss) in zip(ops_small, small):
+            single.set_mesh(c.xn, c.yn, c.zn)
+            b, _ = single.build_rhs(5, [ss[0]])
+            B.add_system(o, b)
+            plan.append((c, c.omega, c.model, c.cells, b, False))
+        reps, secs = B.solve(1e-9, 5000)
+        print(f"[batch split] {len(reps)} systems {secs:.3f}s iters {[r['iters'] for r in reps]}",
+              flush=True)
+        for q, (c, om, model, cells, b, big) in enumerate(plan):
+            r = reps[q]
+            assert r["status"] == 0 and r["true_rel_res"] < 1.05e-9, (q, r)
+            single.set_mesh(c.xn, c.yn, c.zn)
+            single.set_cells(model, om, cells)
+            rs = single.qmr_solve(b, 1e-9, 5000, hist_cap=0)
+            x, xs = flat4(B.solution(q)), flat4(rs.x)
+            err = np.linalg.norm(x - xs) / np.linalg.norm(xs)
+            if big:
+                assert r["iters"] == rs.iters and err <= 1e-13, (q, r["iters"], rs.iters, err)
+            else:
+                assert abs(r["iters"] - rs.iters) <= max(3, 0.05 * rs.iters) and err < 1e-6
+        # a first solve stops on the recurrence residual, so the true residual
+        # it leaves may sit just above the target: a restart then needs a few
+        # iterations, far fewer than from zero
+        reps2, _ = B.solve(1e-9, 5000, flags=gpu_mod.F_X0)
+        assert all(r["status"] == 0 and r["true_rel_res"] < 1e-9 and r["iters"] <= 10
+                   for r in reps2), reps2
+    single.close()
