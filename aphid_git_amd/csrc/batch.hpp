@@ -173,6 +173,7 @@ __global__ void kb_fin_init_v(BatchDev B, const double* part, int max_it, int hi
 
 // The streaming kernels take the non-temporal hint NTS of qmr.hpp's
 // ldv/stv as a template flag; the batch picks it per kernel by timing.
+// 1/diag loads stay plain, as in the single-system kernels.
 template <typename V, bool NTS = false>
 __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
                            vec<V>* __restrict__ p) {
@@ -182,7 +183,7 @@ __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V
   const c128 k1 = st->k1;
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
   B_LOOP(q, it)
-    stv<NTS>(p + q, dn<V>(up(ldv<NTS>(v + q)) * up(ldv<NTS>(invd + q + io)) - up(ldv<NTS>(p + q)) * k1));
+    stv<NTS>(p + q, dn<V>(up(ldv<NTS>(v + q)) * up(ldv<false>(invd + q + io)) - up(ldv<NTS>(p + q)) * k1));
 }
 
 // batched stencil: one node per thread, work list of (system, x-plane,
@@ -287,7 +288,7 @@ __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __re
   B_LOOP(q, it)
     update_elem<V, NTS>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q), ldv<NTS>(t + q),
                         ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
-                        ldv<NTS>(invd + q + io), eta, beta, k2, rho, d, x, s_, r, v, a);
+                        ldv<false>(invd + q + io), eta, beta, k2, rho, d, x, s_, r, v, a);
   block_sum<3>(a);
   block_partial<3>(a, part);
 }

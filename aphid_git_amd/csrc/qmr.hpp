@@ -132,7 +132,10 @@ __global__ void k_fin_init_v(const double* part, int nb, QmrState* st) {
 // touches 13 Krylov vectors once each, far more than the L2 and the
 // Infinity Cache hold.  Which hint and launch shape is fastest depends on
 // the vector length (HBM channel interleaving), so the solver times the
-// candidates per mesh (stream shapes in aphid_gpu.hip).
+// candidates per mesh (stream shapes in aphid_gpu.hip).  1/diag is the
+// exception: the fused update reads it and the next iteration's p-update
+// reads it again at once, so its loads stay plain (cacheable) under the
+// hint.
 template <bool NT, typename T>
 __device__ __forceinline__ cplx<T> ldv(const cplx<T>* p) {
   if constexpr (NT) {
@@ -162,7 +165,7 @@ __global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       a[u] = ldv<NT>(v + q + u * stride);
-      b[u] = ldv<NT>(invd + q + u * stride);
+      b[u] = ldv<false>(invd + q + u * stride);
       c[u] = ldv<NT>(p + q + u * stride);
     }
 #pragma unroll
@@ -170,7 +173,7 @@ __global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict
       stv<NT>(p + q + u * stride, dn<V>(up(a[u]) * up(b[u]) - up(c[u]) * k1));
   }
   for (; q < n; q += stride)
-    stv<NT>(p + q, dn<V>(up(ldv<NT>(v + q)) * up(ldv<NT>(invd + q)) - up(ldv<NT>(p + q)) * k1));
+    stv<NT>(p + q, dn<V>(up(ldv<NT>(v + q)) * up(ldv<false>(invd + q)) - up(ldv<NT>(p + q)) * k1));
 }
 
 // C: rho' = ||t - beta v|| and the step-4 scalars (:332-345)
@@ -268,14 +271,14 @@ __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, v
       const long long e = q + u * stride;
       P_[u] = ldv<NT>(p + e); D_[u] = ldv<NT>(d + e); X_[u] = ldv<NT>(x + e);
       T_[u] = ldv<NT>(t + e); S_[u] = ldv<NT>(s + e); R_[u] = ldv<NT>(r + e);
-      Vv[u] = ldv<NT>(v + e); I_[u] = ldv<NT>(invd + e);
+      Vv[u] = ldv<NT>(v + e); I_[u] = ldv<false>(invd + e);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) body(q + u * stride, P_[u], D_[u], X_[u], T_[u], S_[u], R_[u], Vv[u], I_[u]);
   }
   for (; q < n; q += stride)
     body(q, ldv<NT>(p + q), ldv<NT>(d + q), ldv<NT>(x + q), ldv<NT>(t + q), ldv<NT>(s + q),
-         ldv<NT>(r + q), ldv<NT>(v + q), ldv<NT>(invd + q));
+         ldv<NT>(r + q), ldv<NT>(v + q), ldv<false>(invd + q));
   block_sum<3>(a);
   block_partial<3>(a, part);
 }
