@@ -293,8 +293,8 @@ int conv_bwd(aphid_unet* n, int l, const bf16_t* in_override, bool need_dx) {
   d.vper = L.vper;
   const dim3 grid((unsigned)((L.kpad + 255) / 256), (unsigned)L.nsplit);
   UCK(launch_dw(sp.ks, sp.cinp, sp.cout, grid, d, n->s));
-  const long long nw = (long long)sp.cout * sp.cin * L.nt;
-  hipLaunchKernelGGL(kc_dw_reduce, dim3((unsigned)((nw + 63) / 64)), dim3(kBlock), 0, n->s,
+  const long long ne = (long long)L.coutp * L.kpad;  // entries of one partial
+  hipLaunchKernelGGL(kc_dw_reduce, dim3((unsigned)((ne + 63) / 64)), dim3(kBlock), 0, n->s,
                      (const float*)n->dwpart, L.nsplit, L.coutp, L.kpad, sp.cout, sp.cin, sp.cinp,
                      L.nt, n->g + L.woff);
   UCK(hipGetLastError());

@@ -490,22 +490,22 @@ __global__ __launch_bounds__(256) void kc_dw3l(DwArgs a, int nbricks) {
 }
 
 // grad[(co*cin_real + ci)*NT + tap] += sum_s part[s][co][tap*cinp + ci]
-// A block of 256 threads owns 64 consecutive outputs; the 4 threads of an
-// output sum the splits congruent to their index mod 4, then combine in a
-// fixed order (deterministic).
+// A block of 256 threads owns 64 consecutive entries of the [COUTP][KPAD]
+// partial layout, so every wave's loads are one contiguous 256-byte run;
+// the 4 threads of an entry sum the splits congruent to their index mod 4,
+// then combine in a fixed order (the same order for every entry, so the
+// result is deterministic), and the weight (co, ci, tap) of a real entry
+// receives the total.  Padding entries (co >= cout, ci >= cin_real, tap >=
+// nt) are summed and dropped.
 __global__ __launch_bounds__(256) void kc_dw_reduce(const float* __restrict__ part, int nsplit,
                                                     int coutp, int kpad, int cout, int cin_real,
                                                     int cinp, int nt, float* __restrict__ grad) {
   __shared__ float red[4][64];
-  const int n = cout * cin_real * nt;
   const int o = threadIdx.x % 64, g = threadIdx.x / 64;
-  const int q = blockIdx.x * 64 + o;
+  const size_t stride = (size_t)coutp * kpad;
+  const size_t e = (size_t)blockIdx.x * 64 + o;
   float acc = 0.f;
-  size_t e = 0;
-  if (q < n) {
-    const int tap = q % nt, ci = (q / nt) % cin_real, co = q / (nt * cin_real);
-    e = (size_t)co * kpad + tap * cinp + ci;
-    const size_t stride = (size_t)coutp * kpad;
+  if (e < stride) {
     int s = g;
     for (; s + 12 < nsplit; s += 16)
       acc += (part[s * stride + e] + part[(s + 4) * stride + e]) +
@@ -514,7 +514,11 @@ __global__ __launch_bounds__(256) void kc_dw_reduce(const float* __restrict__ pa
   }
   red[g][o] = acc;
   __syncthreads();
-  if (g == 0 && q < n) grad[q] += (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
+  if (g == 0 && e < stride) {
+    const int co = (int)(e / kpad), k = (int)(e % kpad), tap = k / cinp, ci = k % cinp;
+    if (co < cout && tap < nt && ci < cin_real)
+      grad[((size_t)co * cin_real + ci) * nt + tap] += (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
+  }
 }
 
 // ---------------------------------------------------------------------------
