@@ -303,17 +303,10 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   double eacc_re = 0.0, eacc_im = 0.0;
 
   for (int i = i0; i < i1; ++i) {
-    // plane i+1 of the own column: the x-neighbours of this step
-#pragma unroll
-    for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
-    const C k2x_n = ldc(K2, 0, i + 1), k2p_n = ldc(K2, 3, i + 1);
-
-    // stage plane i: own values, then the halo
-#pragma unroll
-    for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
+    // stage plane i: the loads this step needs (own k2y, k2z and the
+    // halo) go out first, since waits on loads complete in issue order
     sm[4][ol] = ldc(K2, 1, i);
     sm[5][ol] = ldc(K2, 2, i);
-    sm[6][ol] = k2p_c;
     if (hv) {
       const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
@@ -321,6 +314,13 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
 #pragma unroll
       for (int c = 1; c < 4; ++c) sm[3 + c][hl] = hin ? ldo(K2, c * nnb + g) : zero;
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
+    sm[6][ol] = k2p_c;
+    // plane i+1 of the own column (the x-neighbours of the next step)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
+    const C k2x_n = ldc(K2, 0, i + 1), k2p_n = ldc(K2, 3, i + 1);
     __syncthreads();
 
     if (own) {
