@@ -1903,6 +1903,18 @@ namespace {
 constexpr int kBT = 256;  // batched stencil block (flat in-plane chunk)
 
 void batch_free_device(aphid_batch* B) {
+  // operator contexts that borrowed slices of the flat vectors let go of
+  // them (and of graphs captured on them) before the vectors are freed
+  for (aphid_ctx* c : B->ops) {
+    if (!c || !c->vborrowed) continue;
+    for (int p = 0; p < 2; ++p) {
+      if (c->gexec[p]) hipGraphExecDestroy(c->gexec[p]);
+      c->gexec[p] = nullptr;
+      c->gvec[p] = nullptr;
+    }
+    for (auto& v : c->V) v = nullptr;
+    c->vborrowed = false;
+  }
   if (B->gexec) hipGraphExecDestroy(B->gexec), B->gexec = nullptr;
   if (B->gexec32) hipGraphExecDestroy(B->gexec32), B->gexec32 = nullptr;
   for (auto& v : B->F)
