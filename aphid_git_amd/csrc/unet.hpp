@@ -170,16 +170,30 @@ __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
   const int b = blockIdx.x;
   const int z0 = (b % nbz) * 16, y0 = ((b / nbz) % nby) * TY, x0 = (b / (nbz * nby)) * TX;
 
-  // stage the brick with its halo (coalesced: pieces of a voxel, then z)
-  for (int p = tid; p < NV * PCS; p += 256) {
-    const int vox = p / PCS, pc = p - vox * PCS;
-    const int lz = vox % BZ, ly = (vox / BZ) % BY, lx = vox / (BZ * BY);
-    const int x = x0 + lx - 1, y = y0 + ly - 1, z = z0 + lz - 1;
-    u32x4 q = {0u, 0u, 0u, 0u};
-    if (x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z)
-      q = *reinterpret_cast<const u32x4*>(a.in + (size_t)((x * a.Y + y) * a.Z + z) * a.in_ld +
-                                          a.in_off + 8 * pc);
-    *reinterpret_cast<u32x4*>(&lds[vox * CIN + 8 * pc]) = q;
+  // stage the brick with its halo (coalesced: pieces of a voxel, then z);
+  // SB loads are in flight before their LDS stores
+  constexpr int SB = 4;
+  for (int p0 = tid; p0 < NV * PCS; p0 += SB * 256) {
+    u32x4 q[SB];
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int p = p0 + u * 256;
+      const int vox = p / PCS, pc = p - vox * PCS;
+      const int lz = vox % BZ, ly = (vox / BZ) % BY, lx = vox / (BZ * BY);
+      const int x = x0 + lx - 1, y = y0 + ly - 1, z = z0 + lz - 1;
+      const bool in = p < NV * PCS && x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z;
+      const size_t src = in ? (size_t)((x * a.Y + y) * a.Z + z) * a.in_ld + a.in_off + 8 * pc : 0;
+      q[u] = *reinterpret_cast<const u32x4*>(a.in + src);
+      if (!in) q[u] = u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      const int p = p0 + u * 256;
+      if (p < NV * PCS) {
+        const int vox = p / PCS, pc = p - vox * PCS;
+        *reinterpret_cast<u32x4*>(&lds[vox * CIN + 8 * pc]) = q[u];
+      }
+    }
   }
   __syncthreads();
 
@@ -189,14 +203,20 @@ __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
 #pragma unroll
     for (int n = 0; n < NB; ++n) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (int ks = 0; ks < KPAD / 32; ++ks) {
+  // the weight fragments of step ks+1 are loaded while step ks runs its
+  // MFMAs, so the global-load latency is paid once, not once per step
+  constexpr int NK = KPAD / 32;
+  bf16x8 bw[NB], bwn[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) bw[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + 8 * g);
+  for (int ks = 0; ks < NK; ++ks) {
     const int kk = ks * 32 + 8 * g;
     const int tap = kk / CIN, ci = kk - tap * CIN;
     const bool tv = tap < 27;
     const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
-    bf16x8 bw[NB];
+    const int kn = ks + 1 < NK ? kk + 32 : kk;  // the last step reloads its own
 #pragma unroll
-    for (int n = 0; n < NB; ++n) bw[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + kk);
+    for (int n = 0; n < NB; ++n) bwn[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + kn);
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
       const int row = w * MT + t, tx = row / TY, ty = row - tx * TY;
@@ -206,6 +226,8 @@ __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
       for (int n = 0; n < NB; ++n)
         acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[n], acc[t][n], 0, 0, 0);
     }
+#pragma unroll
+    for (int n = 0; n < NB; ++n) bw[n] = bwn[n];
   }
 #pragma unroll
   for (int t = 0; t < MT; ++t) {
@@ -405,33 +427,62 @@ __global__ __launch_bounds__(256) void kc_dw3l(DwArgs a, int nbricks) {
       const int z0b = (br % nbz) * 16, y0b = ((br / nbz) % nby) * TY, x0b = (br / (nbz * nby)) * TX;
       // dY of the brick's voxels, transposed; zero outside the grid and for
       // padded output channels
-      for (int p = tid; p < NVOX * (COUTP / 8); p += 256) {
-        const int vox = p % NVOX, pc = p / NVOX;
-        const int row = vox / 16, z = z0b + vox % 16, x = x0b + row / TY, y = y0b + row % TY;
-        u32x4 q = {0u, 0u, 0u, 0u};
-        if (8 * pc < COUT && x < a.X && y < a.Y && z < a.Z)
-          q = *reinterpret_cast<const u32x4*>(a.gy + (size_t)((x * a.Y + y) * a.Z + z) * COUT + 8 * pc);
+      // (SB loads in flight before their LDS stores, in both staging loops)
+      constexpr int SB = 4;
+      for (int p0 = tid; p0 < NVOX * (COUTP / 8); p0 += SB * 256) {
+        u32x4 qs[SB];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          gt[(8 * pc + 2 * e) * NVOX + vox] = (bf16_t)(q[e] & 0xffffu);
-          gt[(8 * pc + 2 * e + 1) * NVOX + vox] = (bf16_t)(q[e] >> 16);
+        for (int u = 0; u < SB; ++u) {
+          const int p = p0 + u * 256;
+          const int vox = p % NVOX, pc = p / NVOX;
+          const int row = vox / 16, z = z0b + vox % 16, x = x0b + row / TY, y = y0b + row % TY;
+          const bool in = p < NVOX * (COUTP / 8) && 8 * pc < COUT && x < a.X && y < a.Y && z < a.Z;
+          qs[u] = *reinterpret_cast<const u32x4*>(
+              a.gy + (in ? (size_t)((x * a.Y + y) * a.Z + z) * COUT + 8 * pc : 0));
+          if (!in) qs[u] = u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int p = p0 + u * 256;
+          if (p >= NVOX * (COUTP / 8)) continue;
+          const int vox = p % NVOX, pc = p / NVOX;
+          const u32x4 q = qs[u];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            gt[(8 * pc + 2 * e) * NVOX + vox] = (bf16_t)(q[e] & 0xffffu);
+            gt[(8 * pc + 2 * e + 1) * NVOX + vox] = (bf16_t)(q[e] >> 16);
+          }
         }
       }
       // the input brick with its halo, transposed; zero outside the grid
       constexpr int PCS = CIN / 8;
-      for (int p = tid; p < BX * BY * ROW * PCS; p += 256) {
-        const int pc = p % PCS, vox = p / PCS;
-        const int lz = vox % ROW, ly = (vox / ROW) % BY, lx = vox / (ROW * BY);
-        const int x = x0b + lx - 1, y = y0b + ly - 1, z = z0b + lz - 1;
-        u32x4 q = {0u, 0u, 0u, 0u};
-        if (x >= 0 && x < a.X && y >= 0 && y < a.Y && z >= 0 && z < a.Z)
-          q = *reinterpret_cast<const u32x4*>(a.in + (size_t)((x * a.Y + y) * a.Z + z) * a.in_ld +
-                                              a.in_off + 8 * pc);
-        const int base = (lx * BY + ly) * BZP + lz;
+      for (int p0 = tid; p0 < BX * BY * ROW * PCS; p0 += SB * 256) {
+        u32x4 qs[SB];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          bt[(8 * pc + 2 * e) * BPL + base] = (bf16_t)(q[e] & 0xffffu);
-          bt[(8 * pc + 2 * e + 1) * BPL + base] = (bf16_t)(q[e] >> 16);
+        for (int u = 0; u < SB; ++u) {
+          const int p = p0 + u * 256;
+          const int pc = p % PCS, vox = p / PCS;
+          const int lz = vox % ROW, ly = (vox / ROW) % BY, lx = vox / (ROW * BY);
+          const int x = x0b + lx - 1, y = y0b + ly - 1, z = z0b + lz - 1;
+          const bool in = p < BX * BY * ROW * PCS && x >= 0 && x < a.X && y >= 0 && y < a.Y &&
+                          z >= 0 && z < a.Z;
+          qs[u] = *reinterpret_cast<const u32x4*>(
+              a.in + (in ? (size_t)((x * a.Y + y) * a.Z + z) * a.in_ld + a.in_off + 8 * pc : 0));
+          if (!in) qs[u] = u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int p = p0 + u * 256;
+          if (p >= BX * BY * ROW * PCS) continue;
+          const int pc = p % PCS, vox = p / PCS;
+          const int lz = vox % ROW, ly = (vox / ROW) % BY, lx = vox / (ROW * BY);
+          const int base = (lx * BY + ly) * BZP + lz;
+          const u32x4 q = qs[u];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            bt[(8 * pc + 2 * e) * BPL + base] = (bf16_t)(q[e] & 0xffffu);
+            bt[(8 * pc + 2 * e + 1) * BPL + base] = (bf16_t)(q[e] >> 16);
+          }
         }
       }
       __syncthreads();
