@@ -13,7 +13,8 @@
 //   - the x-neighbours (i-1, i+1) of every field live in a per-thread
 //     register queue; the in-plane neighbours (j+-1, k+-1) come from the
 //     current plane staged in LDS with a one-cell halo (2*(TJ+TK) halo
-//     cells, corners are never needed by a 7-point stencil);
+//     cells, corners are never needed by a 7-point stencil; a halo cell
+//     holds the four fields and the one k^2 grid its side needs);
 //   - outside the domain the staged value is 0, which is what the
 //     reference's clamped-index-times-zero-coefficient terms contribute.
 #pragma once
@@ -287,6 +288,10 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   }
   const int gj = bj * TJ + hj, gk = bk * TK + hk;
   const bool hin = hv && gj >= 0 && gj < ny && gk >= 0 && gk < nz;
+  // the one k^2 grid a halo cell serves: the j-1 row feeds k2y(j-1), the
+  // k-1 column k2z(k-1), the j+1 row and the k+1 column k2p (rows4 reads
+  // no other k^2 value off the tile)
+  const int hc = hj == -1 ? 1 : (hk == -1 ? 2 : 3);
   const unsigned hcolb = hin ? ((unsigned)gj * nz + gk) * CS : 0u;
   const int hl = (hj + 1) * LK + (hk + 1);
   const int ol = (tj + 1) * LK + (tk + 1);
@@ -311,8 +316,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
       const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
       for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : zero;
-#pragma unroll
-      for (int c = 1; c < 4; ++c) sm[3 + c][hl] = hin ? ldo(K2, c * nnb + g) : zero;
+      sm[3 + hc][hl] = hin ? ldo(K2, hc * nnb + g) : zero;
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
