@@ -729,11 +729,13 @@ int enqueue_iteration(aphid_ctx* c) {
   return 0;
 }
 
-// iterations per captured graph: about 3 ms of work per replay
+// iterations per captured graph: about 10 ms of work per replay, so that
+// the host's state read between replays (an idle gap of a few tens of
+// microseconds) costs well under 1%; iterations past the stop return at once
 int graph_iters(aphid_ctx* c, int prec) {
   const double bytes = 22.0 * 4.0 * (double)c->nn * (prec ? 8.0 : 16.0);
   const double t_it = std::max(bytes / 4.0e12, 25e-6);
-  return std::max(4, std::min(128, (int)std::ceil(3e-3 / t_it)));
+  return std::max(4, std::min(128, (int)std::ceil(1e-2 / t_it)));
 }
 
 // APHID_NO_GRAPH=1 enqueues the iterations directly on the stream instead
@@ -2315,7 +2317,7 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
   if (rc) return rc;
   if (batch_all_done(B)) return 0;
   const double bytes = 22.0 * (f64 ? 16.0 : 8.0) * (double)B->n4_tot;
-  const int k = std::max(4, std::min(128, (int)std::ceil(3e-3 / std::max(bytes / 4.0e12, 25e-6))));
+  const int k = std::max(4, std::min(128, (int)std::ceil(1e-2 / std::max(bytes / 4.0e12, 25e-6))));
   hipGraphExec_t& gx = f64 ? B->gexec : B->gexec32;
   int& gk = f64 ? B->giters : B->giters32;
   if (!gx || gk != k) {
