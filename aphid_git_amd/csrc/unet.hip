@@ -274,6 +274,7 @@ int conv_bwd(aphid_unet* n, int l, const bf16_t* in_override, bool need_dx) {
   const int V = lv_dims(n, sp.level, X, Y, Z);
   const int C = sp.cout;
   const long long nel = (long long)V * C;
+  if (C & (C - 1)) return aphid_fail(APHID_E_STATE, "relu backward needs a power-of-two channel count");
   const int nbb = (int)std::min<long long>(kBiasBlocks, (nel + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(kc_relu_bwd, dim3(nbb), dim3(kBlock), 0, n->s, V, C,
                      (const float*)n->dbuf[sp.out], kBufs[sp.out].ch, sp.out_off,

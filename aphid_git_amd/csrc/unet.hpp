@@ -582,13 +582,15 @@ __global__ __launch_bounds__(256) void kc_relu_bwd(int V, int C, const float* __
                                                    int y_ld, int y_off, bf16_t* __restrict__ gy,
                                                    float* __restrict__ part) {
   __shared__ float red[256];
-  const int tid = threadIdx.x, c = tid % C;
+  // C is a power of two (8..64): voxel index by shift, 32-bit arithmetic
+  const int lc = __builtin_ctz((unsigned)C);
+  const int tid = threadIdx.x, c = tid & (C - 1);
   float s = 0.f;
-  const long long n = (long long)V * C;
-  for (long long q = (long long)blockIdx.x * 256 + tid; q < n; q += (long long)gridDim.x * 256) {
-    const long long v = q / C;
-    float d = dy[v * dy_ld + dy_off + c];
-    if (y && !(bf2f(y[v * y_ld + y_off + c]) > 0.f)) d = 0.f;
+  const int n = V * C;
+  for (int q = blockIdx.x * 256 + tid; q < n; q += gridDim.x * 256) {
+    const int v = q >> lc;
+    float d = dy[(size_t)v * dy_ld + dy_off + c];
+    if (y && !(bf2f(y[(size_t)v * y_ld + y_off + c]) > 0.f)) d = 0.f;
     const bf16_t h = f2bf(d);
     gy[q] = h;
     s += bf2f(h);
@@ -626,65 +628,93 @@ __device__ __forceinline__ int fine_of(int xc, int yc, int zc, int a, int Y, int
   return ((2 * xc + (a >> 2)) * Y + (2 * yc + ((a >> 1) & 1))) * Z + (2 * zc + (a & 1));
 }
 
+// The four kernels below move 16 bytes per thread and step: 8 bf16 or 4
+// FP32 channels of one voxel (C, the leading dimensions and the channel
+// offsets are multiples of 8), with 32-bit index arithmetic.  Per channel
+// the arithmetic and its order are those of an element-wise loop.
+__device__ __forceinline__ float bf_lo(unsigned w) { return bf2f((bf16_t)(w & 0xffffu)); }
+__device__ __forceinline__ float bf_hi(unsigned w) { return bf2f((bf16_t)(w >> 16)); }
+__device__ __forceinline__ unsigned bf_pack(float lo, float hi) {
+  return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+}
+
 __global__ void kc_pool(int X, int Y, int Z, int C, const bf16_t* __restrict__ in, int in_ld,
                         int in_off, bf16_t* __restrict__ out) {
-  const int Xc = X / 2, Yc = Y / 2, Zc = Z / 2;
-  const long long n = (long long)Xc * Yc * Zc * C;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(q % C);
-    const int vc = (int)(q / C);
+  const int Xc = X / 2, Yc = Y / 2, Zc = Z / 2, P = C / 8;
+  const int n = Xc * Yc * Zc * P;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int pc = q % P, vc = q / P;
     const int zc = vc % Zc, yc = (vc / Zc) % Yc, xc = vc / (Zc * Yc);
-    float s = 0.f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < 8; ++a) s += bf2f(in[(size_t)fine_of(xc, yc, zc, a, Y, Z) * in_ld + in_off + c]);
-    out[q] = f2bf(s * 0.125f);
+    for (int a = 0; a < 8; ++a) {
+      const u32x4 w = *reinterpret_cast<const u32x4*>(
+          in + (size_t)fine_of(xc, yc, zc, a, Y, Z) * in_ld + in_off + 8 * pc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        s[2 * e] += bf_lo(w[e]);
+        s[2 * e + 1] += bf_hi(w[e]);
+      }
+    }
+    u32x4 o;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) o[e] = bf_pack(s[2 * e] * 0.125f, s[2 * e + 1] * 0.125f);
+    *reinterpret_cast<u32x4*>(out + (size_t)vc * C + 8 * pc) = o;
   }
 }
 
 // d_in[fine][c] += d_out[coarse][c] / 8
 __global__ void kc_pool_bwd(int X, int Y, int Z, int C, const float* __restrict__ dout,
                             float* __restrict__ din, int din_ld, int din_off) {
-  const long long n = (long long)X * Y * Z * C;
-  const int Yc = Y / 2, Zc = Z / 2;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(q % C);
-    const int v = (int)(q / C);
+  const int Yc = Y / 2, Zc = Z / 2, P = C / 4;
+  const int n = X * Y * Z * P;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int pc = q % P, v = q / P;
     const int z = v % Z, y = (v / Z) % Y, x = v / (Z * Y);
     const int vc = ((x / 2) * Yc + y / 2) * Zc + z / 2;
-    din[(size_t)v * din_ld + din_off + c] += 0.125f * dout[(size_t)vc * C + c];
+    const float4 d = *reinterpret_cast<const float4*>(dout + (size_t)vc * C + 4 * pc);
+    float4* o = reinterpret_cast<float4*>(din + (size_t)v * din_ld + din_off + 4 * pc);
+    float4 cur = *o;
+    cur.x += 0.125f * d.x;
+    cur.y += 0.125f * d.y;
+    cur.z += 0.125f * d.z;
+    cur.w += 0.125f * d.w;
+    *o = cur;
   }
 }
 
 __global__ void kc_up(int X, int Y, int Z, int C, const bf16_t* __restrict__ in,
                       bf16_t* __restrict__ out, int out_ld, int out_off) {
-  const long long n = (long long)X * Y * Z * C;
-  const int Yc = Y / 2, Zc = Z / 2;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(q % C);
-    const int v = (int)(q / C);
+  const int Yc = Y / 2, Zc = Z / 2, P = C / 8;
+  const int n = X * Y * Z * P;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int pc = q % P, v = q / P;
     const int z = v % Z, y = (v / Z) % Y, x = v / (Z * Y);
     const int vc = ((x / 2) * Yc + y / 2) * Zc + z / 2;
-    out[(size_t)v * out_ld + out_off + c] = in[(size_t)vc * C + c];
+    *reinterpret_cast<u32x4*>(out + (size_t)v * out_ld + out_off + 8 * pc) =
+        *reinterpret_cast<const u32x4*>(in + (size_t)vc * C + 8 * pc);
   }
 }
 
 // d_in[coarse][c] = sum of the 8 children's d_out
 __global__ void kc_up_bwd(int X, int Y, int Z, int C, const float* __restrict__ dout, int dout_ld,
                           int dout_off, float* __restrict__ din) {
-  const int Xc = X / 2, Yc = Y / 2, Zc = Z / 2;
-  const long long n = (long long)Xc * Yc * Zc * C;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(q % C);
-    const int vc = (int)(q / C);
+  const int Xc = X / 2, Yc = Y / 2, Zc = Z / 2, P = C / 4;
+  const int n = Xc * Yc * Zc * P;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int pc = q % P, vc = q / P;
     const int zc = vc % Zc, yc = (vc / Zc) % Yc, xc = vc / (Zc * Yc);
-    float s = 0.f;
+    float4 s = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < 8; ++a) s += dout[(size_t)fine_of(xc, yc, zc, a, Y, Z) * dout_ld + dout_off + c];
-    din[q] = s;
+    for (int a = 0; a < 8; ++a) {
+      const float4 d = *reinterpret_cast<const float4*>(
+          dout + (size_t)fine_of(xc, yc, zc, a, Y, Z) * dout_ld + dout_off + 4 * pc);
+      s.x += d.x;
+      s.y += d.y;
+      s.z += d.z;
+      s.w += d.w;
+    }
+    *reinterpret_cast<float4*>(din + (size_t)vc * C + 4 * pc) = s;
   }
 }
 
