@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
-PARTS = ("solve", "batch", "unet", "warm", "cpu")
+PARTS = ("solve", "shard", "batch", "unet", "warm", "cpu")
 EXTRA_PARTS = ("batch3b", "warm5", "prec4")  # not in the default set (minutes of GPU time)
 
 
@@ -249,6 +249,45 @@ def bench_prec4(freq: float = 1e4, max_it: int = 100000, stall: int = 1000):
     return out
 
 
+def bench_shard(nshard: int = 8):
+    """One GPU's share of config 2 under the x-slab decomposition of a mesh
+    too large for one GPU: an interior slab of ceil(129/nshard) x-planes
+    plus one halo plane per side (129/8 -> 17 + 2 = 19 planes), as its own
+    mesh, with the same y/z extent and cells.  Times the apply and one QMR
+    iteration on a live Krylov state (a capped solve of a random b).  The
+    halo exchange and the scalar all-reduces are not run (one GPU)."""
+    from aphid_git_amd import gpu, meshgen
+
+    base = meshgen.config2()
+    nx = len(base.xn)
+    w = -(-nx // nshard) + 2
+    i0 = (nx - w) // 2
+    cells = {k: np.asfortranarray(v[i0:i0 + w - 1]) for k, v in base.cells.items()}
+    case = meshgen.Case(base.xn[i0:i0 + w].copy(), base.yn, base.zn, base.model, cells, base.freq,
+                        5, [], base.target, base.max_it, "config-2 x-slab")
+    s = gpu.Solver(0)
+    s.setup_case(case)
+    rng = np.random.default_rng(11)
+    b = [np.asfortranarray(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+         for sh in gpu.comp_shapes(*case.shape)]
+    s.load_rhs(b)
+    s.solve_resident(1e-12, 300)
+    t_apply = s.time_apply(0, 20)
+    kt = s.time_iteration(0, 10)
+    tile = s.get_tile(0)
+    s.close()
+    nxs, ny, nz = case.shape
+    M = 4 * nxs * ny * nz  # stored entries (the slab's unknowns plus padding)
+    halo = 2 * 4 * ny * nz * 16
+    return {"workload": f"config-2 x-slab {nxs}x{ny}x{nz} nodes (1/{nshard} of 129 x-planes + "
+                        f"1 halo plane per side), complex128",
+            "entries": M, "tile": list(tile),
+            "apply_us": t_apply * 1e6, "apply_GBps": 3 * M * 16 / t_apply / 1e9,
+            "iteration_us": kt[4] * 1e6, "iteration_GBps": 22 * M * 16 / kt[4] / 1e9,
+            "halo_exchange_bytes_per_apply": halo,
+            "full_mesh_iteration_share_us": None}
+
+
 def bench_unet(reps: int = 5):
     from aphid_git_amd import gpu
 
@@ -398,6 +437,12 @@ def main():
         })
         out["config"].update({"iterations_per_solve": iters, "converged_steps": converged,
                               "true_rel_res_max": max(r.true_rel_res for r in results)})
+    if "shard" in parts:
+        log("[bench] one GPU's share of config 2 (x-slab of 8) ...")
+        out["shard_config2_1of8"] = bench_shard()
+        if "iteration" in out:
+            out["shard_config2_1of8"]["full_mesh_iteration_share_us"] = out["iteration"]["seconds"] * 1e6 / 8
+        log(f"[bench] shard: {out['shard_config2_1of8']}")
     if "batch" in parts:
         log("[bench] batch: 192 systems of the config-1 family ...")
         out["batch_config3a"] = bench_batch()
