@@ -419,7 +419,6 @@ def main():
             # PMC traffic of the same kernel and tile only
             if list(tj.get("tile", [])) == list(tile):
                 traffic = tj.get("stencil_bytes_per_launch")
-        s.close()
         out.update({
             "value": converged / elapsed,
             "ms_per_step": 1e3 * elapsed / args.steps,
@@ -437,6 +436,27 @@ def main():
         })
         out["config"].update({"iterations_per_solve": iters, "converged_steps": converged,
                               "true_rel_res_max": max(r.true_rel_res for r in results)})
+        # end to end (not `value`): host download of the solution, then a
+        # steady-state re-setup of the case (metrics, k^2, diag, RHS) and
+        # the host upload of b, beside the resident solve's wall time
+        t0 = time.perf_counter()
+        s.fetch_solution()
+        t_d2h = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b2 = s.setup_case(case)
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.load_rhs(b2)
+        t_h2d = time.perf_counter() - t0
+        t_solve = elapsed / max(args.steps, 1)
+        total = t_setup + t_h2d + t_solve + t_d2h
+        out["end_to_end"] = {"setup_s": t_setup, "h2d_s": t_h2d, "solve_s": t_solve, "d2h_s": t_d2h,
+                             "total_s": total,
+                             "converged_solves_per_s": (converged / max(args.steps, 1)) / total,
+                             "note": "host arrays in Fortran order; permutation to the device "
+                                     "layout on the device; not part of `value`"}
+        log(f"[bench] end to end: {out['end_to_end']}")
+        s.close()
     if "shard" in parts:
         log("[bench] one GPU's share of config 2 (x-slab of 8) ...")
         out["shard_config2_1of8"] = bench_shard()
