@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
-PARTS = ("solve", "shard", "batch", "unet", "warm", "cpu")
+PARTS = ("solve", "config4", "shard", "batch", "unet", "warm", "cpu")
 EXTRA_PARTS = ("batch3b", "warm5", "prec4")  # not in the default set (minutes of GPU time)
 
 
@@ -247,6 +247,34 @@ def bench_prec4(freq: float = 1e4, max_it: int = 100000, stall: int = 1000):
     log(f"[bench] prec4 mixed: {out['mixed']}")
     s.close()
     return out
+
+
+def bench_config4():
+    """Config 4 (192x192x160 cells, 23.9M unknowns): the HBM figures of the
+    complex128 apply and iteration, and of the complex64 apply, on a live
+    Krylov state (300 iterations of the config's own right-hand side)."""
+    from aphid_git_amd import gpu, meshgen
+
+    case = meshgen.config4()
+    s = gpu.Solver(0)
+    b = s.setup_case(case)
+    s.load_rhs(b)
+    s.solve_resident(TOL, 300)
+    t128 = s.time_apply(0, 20)
+    kt = s.time_iteration(0, 10)
+    t64 = s.time_apply(1, 20)
+    tiles = {"c128": list(s.get_tile(0)), "c64": list(s.get_tile(1))}
+    s.close()
+    M = case.n_unknowns
+
+    def rate(nbytes, t):
+        return {"seconds": t, "algorithmic_bytes": nbytes, "GBps": nbytes / t / 1e9,
+                "frac": nbytes / t / 1e9 / HBM_PEAK_GBS}
+
+    return {"workload": "config4: 192x192x160-cell stretched mesh, 40 blocks, +X dipole, 1 Hz",
+            "unknowns": M, "tiles": tiles,
+            "apply_c128": rate(3 * M * 16, t128), "iteration_c128": rate(22 * M * 16, kt[4]),
+            "apply_c64": rate(3 * M * 8, t64)}
 
 
 def bench_shard(nshard: int = 8):
@@ -457,6 +485,10 @@ def main():
                                      "layout on the device; not part of `value`"}
         log(f"[bench] end to end: {out['end_to_end']}")
         s.close()
+    if "config4" in parts:
+        log("[bench] config 4 apply and iteration ...")
+        out["config4_kernels"] = bench_config4()
+        log(f"[bench] config4: {out['config4_kernels']}")
     if "shard" in parts:
         log("[bench] one GPU's share of config 2 (x-slab of 8) ...")
         out["shard_config2_1of8"] = bench_shard()

@@ -17,6 +17,6 @@ def test_default_parts_and_extra_parts():
     sys.path.insert(0, ROOT)
     import bench
 
-    assert bench.PARTS == ("solve", "shard", "batch", "unet", "warm", "cpu")
+    assert bench.PARTS == ("solve", "config4", "shard", "batch", "unet", "warm", "cpu")
     assert set(bench.EXTRA_PARTS) == {"batch3b", "warm5", "prec4"}
     assert bench.HBM_PEAK_GBS == 8000.0 and bench.BF16_PEAK_TFLOPS == 2500.0
