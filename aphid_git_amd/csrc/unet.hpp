@@ -575,31 +575,48 @@ __global__ __launch_bounds__(256) void kc_dw_reduce(const float* __restrict__ pa
 // ---------------------------------------------------------------------------
 // ReLU backward and bias gradient:
 //   gy[v][c] = bf16(dy[v][c] * (y[v][c] > 0))   (y == nullptr: no ReLU)
-//   part[block][c] = sum over the block's voxels of gy (FP32, pre-rounding)
-// C divides 256, so every thread of the grid-stride loop keeps one channel.
+//   part[block][c] = sum over the block's voxels of gy (FP32, of the
+//   bf16-rounded values)
 __global__ __launch_bounds__(256) void kc_relu_bwd(int V, int C, const float* __restrict__ dy,
                                                    int dy_ld, int dy_off, const bf16_t* __restrict__ y,
                                                    int y_ld, int y_off, bf16_t* __restrict__ gy,
                                                    float* __restrict__ part) {
-  __shared__ float red[256];
-  // C is a power of two (8..64): voxel index by shift, 32-bit arithmetic
-  const int lc = __builtin_ctz((unsigned)C);
-  const int tid = threadIdx.x, c = tid & (C - 1);
-  float s = 0.f;
-  const int n = V * C;
+  // a thread owns 4 consecutive channels (one 16-byte load of dy); C is a
+  // power of two (8..64), so every thread keeps its channels in the
+  // grid-stride loop
+  __shared__ float red[4][256];
+  const int P = C >> 2, lp = __builtin_ctz((unsigned)P);
+  const int tid = threadIdx.x, pc = tid & (P - 1);
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  const int n = V * P;
   for (int q = blockIdx.x * 256 + tid; q < n; q += gridDim.x * 256) {
-    const int v = q >> lc;
-    float d = dy[(size_t)v * dy_ld + dy_off + c];
-    if (y && !(bf2f(y[(size_t)v * y_ld + y_off + c]) > 0.f)) d = 0.f;
-    const bf16_t h = f2bf(d);
-    gy[q] = h;
-    s += bf2f(h);
+    const int v = q >> lp;
+    const float4 d4 = *reinterpret_cast<const float4*>(dy + (size_t)v * dy_ld + dy_off + 4 * pc);
+    float d[4] = {d4.x, d4.y, d4.z, d4.w};
+    if (y) {
+      const uint2 yy = *reinterpret_cast<const uint2*>(y + (size_t)v * y_ld + y_off + 4 * pc);
+      const float yv[4] = {bf2f((bf16_t)(yy.x & 0xffffu)), bf2f((bf16_t)(yy.x >> 16)),
+                           bf2f((bf16_t)(yy.y & 0xffffu)), bf2f((bf16_t)(yy.y >> 16))};
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (!(yv[e] > 0.f)) d[e] = 0.f;
+    }
+    bf16_t h[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      h[e] = f2bf(d[e]);
+      s[e] += bf2f(h[e]);
+    }
+    *reinterpret_cast<uint2*>(gy + (size_t)q * 4) =
+        uint2{(unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16)};
   }
-  red[tid] = s;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) red[e][tid] = s[e];
   __syncthreads();
-  if (tid < C) {
+  if (tid < C) {  // channel c: the threads that own it, in increasing order
+    const int e = tid & 3, p = tid >> 2;
     float t = 0.f;
-    for (int u = tid; u < 256; u += C) t += red[u];
+    for (int u = p; u < 256; u += P) t += red[e][u];
     part[(size_t)blockIdx.x * C + tid] = t;
   }
 }
