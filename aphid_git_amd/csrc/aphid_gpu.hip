@@ -613,7 +613,9 @@ int tune_streams(aphid_ctx* c) {
   ts.c_nm = 1.0;
   ts.max_it = 1 << 30;
   ts.iter = 1;
-  const int reps = c->nn > 2000000 ? 3 : 8;
+  // an even number of sweeps: the fused update alternates between
+  // iterations that store x and r and iterations that do not (k_update)
+  const int reps = c->nn > 2000000 ? 4 : 8;
   auto launch = [&](int k, const Stream (&sh)[SK_COUNT]) -> hipError_t {
     switch (k) {
       case 0:
@@ -636,6 +638,7 @@ int tune_streams(aphid_ctx* c) {
     float tot = 0.f;
     for (int r = 0; r <= reps; ++r)  // the first sweep warms up
       for (int k = 0; k < 4; ++k) {
+        ts.iter = 1 + (r & 1);
         CK(hipMemcpyAsync(c->tst, &ts, sizeof ts, hipMemcpyHostToDevice, c->s));
         CK(hipEventRecord(c->ev0, c->s));
         CK(launch(k, sh));
@@ -888,6 +891,13 @@ int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs)
   int rc = 0;
   if (!c->hst->done) rc = run_loop<V>(c);
   if (rc) return rc;
+  // the loop leaves c->hst current; an odd last iteration left x and r one
+  // step behind
+  if (c->hst->lag) {
+    hipLaunchKernelGGL(k_catch_up<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.x, w.d, w.r, w.s,
+                       n4);
+    CK(hipGetLastError());
+  }
   CK(hipEventRecord(c->ev1, c->s));
   CK(hipEventSynchronize(c->ev1));
   float ms = 0.f;

@@ -286,7 +286,7 @@ __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __re
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
   double a[3] = {0.0, 0.0, 0.0};
   B_LOOP(q, it)
-    update_elem<V, NTS>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q), ldv<NTS>(t + q),
+    update_elem<V, NTS, UPD_EAGER>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q), ldv<NTS>(t + q),
                         ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
                         ldv<false>(invd + q + io), eta, beta, k2, rho, d, x, s_, r, v, a);
   block_sum<3>(a);

@@ -222,8 +222,18 @@ __global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
 }
 
 // one element of step D: d, x, s, r, v updates; accumulates ||r||^2 and
-// v^T M^-1 v (src/aphid_qmr.f90:347-369, 306-311)
-template <typename V, bool NT = false>
+// v^T M^-1 v (src/aphid_qmr.f90:347-369, 306-311).
+// UPD_EAGER stores x and r every iteration.  The single-system solve stores
+// them every second iteration instead: an odd iteration (UPD_SKIP) leaves
+// them one step behind (x_{k-1}, r_{k-1} stay in memory; r_k is formed for
+// its norm only), and the even one after it (UPD_CATCHUP) first rebuilds
+// x_{k-1} = x_{k-2} + d_{k-1} and r_{k-1} = r_{k-2} - s_{k-1} in registers
+// from the d and s it reads anyway, with the operations storing them would
+// have taken, so every value is bit-for-bit the one of the eager
+// recurrence.  A solve that stops after an odd iteration is completed by
+// k_catch_up.
+enum UpdMode { UPD_EAGER, UPD_SKIP, UPD_CATCHUP };
+template <typename V, bool NT, int MODE>
 __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_,
                                             vec<V> sq, vec<V> rq, vec<V> vq, vec<V> iq, c128 eta,
                                             c128 beta, double k2, double rho,
@@ -233,11 +243,13 @@ __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, v
   const c128 tq = up(tq_);
   const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
   stv<NT>(d + q, ddv);
-  stv<NT>(x + q, dn<V>(up(xq) + up(ddv)));
+  if (MODE == UPD_EAGER) stv<NT>(x + q, dn<V>(up(xq) + up(ddv)));
+  if (MODE == UPD_CATCHUP) stv<NT>(x + q, dn<V>(up(dn<V>(up(xq) + up(dq))) + up(ddv)));
   const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
   stv<NT>(s + q, ssv);
-  const vec<V> rrv = dn<V>(up(rq) - up(ssv));
-  stv<NT>(r + q, rrv);
+  const vec<V> rprev = MODE == UPD_CATCHUP ? dn<V>(up(rq) - up(sq)) : rq;
+  const vec<V> rrv = dn<V>(up(rprev) - up(ssv));
+  if (MODE != UPD_SKIP) stv<NT>(r + q, rrv);
   const vec<V> vt = dn<V>(tq - up(vq) * beta);
   const vec<V> vv = dn<V>(divr(up(vt), rho));
   stv<NT>(v + q, vv);
@@ -250,26 +262,29 @@ __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, v
 
 // D: d = eta p + k2 d; x += d; s = eta t + k2 s; r -= s; v = (t - beta v)/rho';
 // ||r|| and the stop test (:347-390), then del = v^T M^-1 v and k1 for the
-// next iteration (:306-313).
-template <typename V, int U = 2, bool NT = false>
-__global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x,
-                         const vec<V>* __restrict__ t, vec<V>* __restrict__ s, vec<V>* __restrict__ r,
-                         vec<V>* __restrict__ v, const vec<V>* __restrict__ invd, long long n,
-                         const QmrState* st, double* part) {
-  if (st->done) return;
+// next iteration (:306-313).  x and r are stored on even iterations only
+// (update_elem); st->lag records whether this iteration left them behind.
+template <typename V, int U, bool NT, int MODE>
+__device__ __forceinline__ void update_pass(const vec<V>* __restrict__ p, vec<V>* __restrict__ d,
+                                            vec<V>* __restrict__ x, const vec<V>* __restrict__ t,
+                                            vec<V>* __restrict__ s, vec<V>* __restrict__ r,
+                                            vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
+                                            long long n, const QmrState* st, double (&a)[3]) {
   const c128 eta = st->eta, beta = st->beta;
   const double k2 = st->k2s, rho = st->rho_np;
-  double a[3] = {0.0, 0.0, 0.0};
   auto body = [&](long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_, vec<V> sq, vec<V> rq,
                   vec<V> vq, vec<V> iq) {
-    update_elem<V, NT>(q, pq, dq, xq, tq_, sq, rq, vq, iq, eta, beta, k2, rho, d, x, s, r, v, a);
+    update_elem<V, NT, MODE>(q, pq, dq, xq, tq_, sq, rq, vq, iq, eta, beta, k2, rho, d, x, s, r, v,
+                             a);
   };
+  const vec<V> none{};
   UNROLLED_LOOP(U, q, stride, n) {
     vec<V> P_[U], D_[U], X_[U], T_[U], S_[U], R_[U], Vv[U], I_[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long long e = q + u * stride;
-      P_[u] = ldv<NT>(p + e); D_[u] = ldv<NT>(d + e); X_[u] = ldv<NT>(x + e);
+      P_[u] = ldv<NT>(p + e); D_[u] = ldv<NT>(d + e);
+      X_[u] = MODE != UPD_SKIP ? ldv<NT>(x + e) : none;
       T_[u] = ldv<NT>(t + e); S_[u] = ldv<NT>(s + e); R_[u] = ldv<NT>(r + e);
       Vv[u] = ldv<NT>(v + e); I_[u] = ldv<false>(invd + e);
     }
@@ -277,10 +292,38 @@ __global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, v
     for (int u = 0; u < U; ++u) body(q + u * stride, P_[u], D_[u], X_[u], T_[u], S_[u], R_[u], Vv[u], I_[u]);
   }
   for (; q < n; q += stride)
-    body(q, ldv<NT>(p + q), ldv<NT>(d + q), ldv<NT>(x + q), ldv<NT>(t + q), ldv<NT>(s + q),
-         ldv<NT>(r + q), ldv<NT>(v + q), ldv<false>(invd + q));
+    body(q, ldv<NT>(p + q), ldv<NT>(d + q), MODE != UPD_SKIP ? ldv<NT>(x + q) : none, ldv<NT>(t + q),
+         ldv<NT>(s + q), ldv<NT>(r + q), ldv<NT>(v + q), ldv<false>(invd + q));
+}
+
+template <typename V, int U = 2, bool NT = false>
+__global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x,
+                         const vec<V>* __restrict__ t, vec<V>* __restrict__ s, vec<V>* __restrict__ r,
+                         vec<V>* __restrict__ v, const vec<V>* __restrict__ invd, long long n,
+                         QmrState* st, double* part) {
+  if (st->done) return;
+  // iteration 1 of a solve is odd: x0 and r0 are in memory
+  const bool full = (st->iter & 1) == 0;
+  double a[3] = {0.0, 0.0, 0.0};
+  if (full)
+    update_pass<V, U, NT, UPD_CATCHUP>(p, d, x, t, s, r, v, invd, n, st, a);
+  else
+    update_pass<V, U, NT, UPD_SKIP>(p, d, x, t, s, r, v, invd, n, st, a);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->lag = full ? 0 : 1;
   block_sum<3>(a);
   block_partial<3>(a, part);
+}
+
+// completes x and r after a solve whose last iteration left them one step
+// behind: x += d, r -= s, as that iteration would have stored them
+template <typename V>
+__global__ void k_catch_up(vec<V>* __restrict__ x, const vec<V>* __restrict__ d,
+                           vec<V>* __restrict__ r, const vec<V>* __restrict__ s, long long n) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    x[q] = dn<V>(up(x[q]) + up(d[q]));
+    r[q] = dn<V>(up(r[q]) - up(s[q]));
+  }
 }
 // ||r||, history, stop test and counter (src/aphid_qmr.f90:371-390), then
 // the loop-entry scalars of the next iteration (:300-313)

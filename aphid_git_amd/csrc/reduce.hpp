@@ -75,7 +75,8 @@ struct QmrState {
   double k2s;                 // theta_{n-1} c_n
   double rnorm, rhsnorm, target;
   int iter, max_it, status, done;
-  int hist_cap, pad0;
+  int hist_cap;
+  int lag;  // 1: the last update left x and r one iteration behind (k_update)
 };
 
 enum : int { ST_CONV = 0, ST_MAXIT = 1, ST_BREAK = 2 };

@@ -43,6 +43,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Vector passes (reads + writes of one 4*nn complex vector) of a QMR
+# iteration: p-update 4, stencil 3 (u, k^2, K u), ||t - beta v|| 2, fused
+# update 11.5 on average (13 on even iterations, which store x and r; 10 on
+# odd ones, which leave them one step behind).
+UPDATE_PASSES = 11.5
+ITERATION_PASSES = 4 + 3 + 2 + UPDATE_PASSES
 BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
@@ -273,7 +279,7 @@ def bench_config4():
 
     return {"workload": "config4: 192x192x160-cell stretched mesh, 40 blocks, +X dipole, 1 Hz",
             "unknowns": M, "tiles": tiles,
-            "apply_c128": rate(3 * M * 16, t128), "iteration_c128": rate(22 * M * 16, kt[4]),
+            "apply_c128": rate(3 * M * 16, t128), "iteration_c128": rate(ITERATION_PASSES * M * 16, kt[4]),
             "apply_c64": rate(3 * M * 8, t64)}
 
 
@@ -311,7 +317,7 @@ def bench_shard(nshard: int = 8):
                         f"1 halo plane per side), complex128",
             "entries": M, "tile": list(tile),
             "apply_us": t_apply * 1e6, "apply_GBps": 3 * M * 16 / t_apply / 1e9,
-            "iteration_us": kt[4] * 1e6, "iteration_GBps": 22 * M * 16 / kt[4] / 1e9,
+            "iteration_us": kt[4] * 1e6, "iteration_GBps": ITERATION_PASSES * M * 16 / kt[4] / 1e9,
             "halo_exchange_bytes_per_apply": halo,
             "full_mesh_iteration_share_us": None}
 
@@ -428,7 +434,7 @@ def main():
             "p_update": (4 * M * s_bytes, kt[0]),
             "stencil_eps": (B_mv, kt[1]),
             "vt_norm": (2 * M * s_bytes, kt[2]),
-            "fused_update": (13 * M * s_bytes, kt[3]),
+            "fused_update": (UPDATE_PASSES * M * s_bytes, kt[3]),
         }
         kernels = {k: {"algorithmic_bytes": int(bb), "seconds": t, "GBps": bb / t / 1e9,
                        "frac": bb / t / 1e9 / HBM_PEAK_GBS} for k, (bb, t) in per_kernel.items()}
@@ -438,7 +444,7 @@ def main():
         copy_probe = {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "seconds": t_copy,
                       "GBps": 2 * copy_bytes / t_copy / 1e9,
                       "frac_of_peak": 2 * copy_bytes / t_copy / 1e9 / HBM_PEAK_GBS}
-        it_bytes = 22 * M * s_bytes
+        it_bytes = ITERATION_PASSES * M * s_bytes
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic_config2.json")
         if os.path.exists(tf):

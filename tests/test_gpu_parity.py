@@ -103,6 +103,25 @@ def test_every_tile_shape_and_variant_gives_the_same_results(gpu):
                 assert relerr(out, ref) < 1e-14
 
 
+@pytest.mark.parametrize("max_it", [1, 2, 3, 4, 7, 8])
+def test_iterate_after_any_iteration_count_matches_oracle(gpu, oracle, max_it):
+    """The fused update stores x and r on even iterations only and the solve
+    completes them after an odd last one: the iterate returned after 1-8
+    iterations is the oracle's (a missing or doubled d_k would be an O(1)
+    error; the bound covers the stencil's multiply-add rounding)."""
+    case = CASES["blocks_23x31x19"]()
+    print(f"[lagged x/r] max_it {max_it}", flush=True)
+    b = gpu.setup_case(case)
+    S = oracle.System(case)
+    r = gpu.qmr_solve(b, 1e-30, max_it)
+    xo, rep, _ = S.solve(tol=1e-30, max_it=max_it)
+    assert r.status == 1 and rep.status == 1
+    assert relerr(flat4(r.x), flat4(xo)) < 1e-11
+    # the true residual of the returned x matches the recurrence's ||r||
+    # (the recurrence residual is what the stop test reads)
+    assert abs(r.true_rel_res * r.rhsnorm - r.rnorm) <= 1e-8 * r.rhsnorm
+
+
 @pytest.mark.parametrize("path", golden_files("rhs_"), ids=os.path.basename)
 def test_rhs_matches_reference(gpu, path):
     case, z = load_golden(path)
