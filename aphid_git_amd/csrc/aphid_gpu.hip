@@ -2104,7 +2104,10 @@ int batch_tune_nt(aphid_batch* B) {
   t1.c_nm = 1.0;
   t1.max_it = 1 << 30;
   t1.iter = 1;
-  const std::vector<QmrState> ts(ns, t1);
+  QmrState t2 = t1;
+  t2.iter = 2;
+  // both kinds of update iteration (x and r stored or not), alternately
+  const std::vector<QmrState> ts[2] = {std::vector<QmrState>(ns, t1), std::vector<QmrState>(ns, t2)};
   BatchDev D = batch_dev(B);
   D.st = B->tst;
   cplx<V>*x, *v, *p, *t, *d, *s, *r;
@@ -2118,12 +2121,13 @@ int batch_tune_nt(aphid_batch* B) {
     v = B->F[2]; p = B->F[3]; t = B->F[4]; d = B->F[5]; s = B->F[6]; r = B->F[7];
     invd = B->invdf; ops = B->dops32;
   }
-  const int reps = B->n4_tot > 8000000 ? 3 : 8;
+  const int reps = B->n4_tot > 8000000 ? 4 : 8;
   auto seq_time = [&](const int (&nt)[SK_COUNT], float* ms) -> int {
     float tot = 0.f;
     for (int rep = 0; rep <= reps; ++rep)
       for (int k = 0; k < 4; ++k) {
-        CK(hipMemcpyAsync(B->tst, ts.data(), sizeof(QmrState) * ns, hipMemcpyHostToDevice, B->s));
+        CK(hipMemcpyAsync(B->tst, ts[rep & 1].data(), sizeof(QmrState) * ns, hipMemcpyHostToDevice,
+                          B->s));
         CK(hipEventRecord(B->ev0, B->s));
         batch_launch<V>(B, D, k, nt, ops, x, v, p, t, d, s, r, invd);
         CK(hipGetLastError());
@@ -2354,7 +2358,12 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
     }
     rc = batch_fetch_states(B);
     if (rc) return rc;
-    if (batch_all_done(B)) return 0;
+    if (batch_all_done(B)) {
+      // systems that stopped after an odd iteration: x and r one step behind
+      hipLaunchKernelGGL(kb_catch_up<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, x, d, r, s_);
+      CK(hipGetLastError());
+      return 0;
+    }
     if (l > max_launch) return fail(APHID_E_STATE, "batched QMR state did not advance");
   }
 }

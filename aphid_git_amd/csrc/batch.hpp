@@ -279,18 +279,44 @@ __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __re
                           vec<V>* __restrict__ r, vec<V>* __restrict__ v,
                           const vec<V>* __restrict__ invd, double* part) {
   const BItem it = b_item(B);
-  const QmrState* st = &B.st[it.s];
+  QmrState* st = &B.st[it.s];
   if (st->done) return;
   const c128 eta = st->eta, beta = st->beta;
   const double k2 = st->k2s, rho = st->rho_np;
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
+  // x and r are stored on even iterations only, as in k_update
+  const bool full = (st->iter & 1) == 0;
+  const vec<V> none{};
   double a[3] = {0.0, 0.0, 0.0};
-  B_LOOP(q, it)
-    update_elem<V, NTS, UPD_EAGER>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q), ldv<NTS>(t + q),
-                        ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
-                        ldv<false>(invd + q + io), eta, beta, k2, rho, d, x, s_, r, v, a);
+  if (full) {
+    B_LOOP(q, it)
+      update_elem<V, NTS, UPD_CATCHUP>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q),
+                                       ldv<NTS>(t + q), ldv<NTS>(s_ + q), ldv<NTS>(r + q),
+                                       ldv<NTS>(v + q), ldv<false>(invd + q + io), eta, beta, k2,
+                                       rho, d, x, s_, r, v, a);
+  } else {
+    B_LOOP(q, it)
+      update_elem<V, NTS, UPD_SKIP>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), none, ldv<NTS>(t + q),
+                                    ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
+                                    ldv<false>(invd + q + io), eta, beta, k2, rho, d, x, s_, r,
+                                    v, a);
+  }
+  if ((int)blockIdx.x == B.bstart[it.s] && threadIdx.x == 0) st->lag = full ? 0 : 1;
   block_sum<3>(a);
   block_partial<3>(a, part);
+}
+
+// x += d, r -= s for the systems whose last iteration left them one step
+// behind (kb_update)
+template <typename V>
+__global__ void kb_catch_up(BatchDev B, vec<V>* __restrict__ x, const vec<V>* __restrict__ d,
+                            vec<V>* __restrict__ r, const vec<V>* __restrict__ s_) {
+  const BItem it = b_item(B);
+  if (!B.st[it.s].lag) return;
+  B_LOOP(q, it) {
+    x[q] = dn<V>(up(x[q]) + up(d[q]));
+    r[q] = dn<V>(up(r[q]) - up(s_[q]));
+  }
 }
 
 __global__ void kb_fin_update(BatchDev B, const double* part) {

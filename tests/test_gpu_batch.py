@@ -69,6 +69,28 @@ def test_uneven_pack_matches_oracle_and_single_solves(gpu_mod, oracle):
     single.close()
 
 
+@pytest.mark.parametrize("max_it", [5, 6])
+def test_batched_iterate_after_odd_and_even_iteration_counts(gpu_mod, oracle, max_it):
+    """The batched update stores x and r on even iterations only and the
+    solve completes them for systems that stopped after an odd one: the
+    iterate after max_it iterations is the oracle's (a missing or doubled
+    d_k would be an O(1) error)."""
+    c = _uneven_cases()[1][0]
+    with gpu_mod.BatchSolver() as B:
+        single = gpu_mod.Solver()
+        single.set_mesh(c.xn, c.yn, c.zn)
+        b, _ = single.build_rhs(5, c.sources)
+        single.close()
+        o = B.add_case_operator(c)
+        B.add_system(o, b)
+        reps, _ = B.solve(1e-30, max_it)
+        assert reps[0]["status"] == 1
+        S = oracle.System(c)
+        xo, rep, _ = S.solve(tol=1e-30, max_it=max_it)
+        xo = flat4(xo)
+        assert np.linalg.norm(flat4(B.solution(0)) - xo) / np.linalg.norm(xo) < 1e-11
+
+
 def test_shared_operators_frequency_sweep_and_warm_restart(gpu_mod):
     """config-1 family: 3 frequencies x 3 dipoles (+X, +Y, +Z at the surface
     centre) share one operator per frequency; then a warm restart from the
