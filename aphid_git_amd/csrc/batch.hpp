@@ -58,6 +58,25 @@ __device__ __forceinline__ BItem b_item(const BatchDev& B) {
 
 #define B_LOOP(q, it) for (long long q = (it).e0 + threadIdx.x; q < (it).e1; q += blockDim.x)
 
+// Two elements per thread and step (q, q + blockDim.x) with all their loads
+// issued before either is used, then the remaining one: twice the loads in
+// flight of B_LOOP, and the same per-thread order of elements (the order of
+// any partial sum is unchanged).  LOAD(e) loads element e into a struct,
+// BODY(e, loaded) uses it.
+template <typename LOAD, typename BODY>
+__device__ __forceinline__ void b_loop2(const BItem& it, LOAD load, BODY body) {
+  long long q = it.e0 + threadIdx.x;
+  const long long step = blockDim.x;
+  for (; q + step < it.e1; q += 2 * step) {
+    const auto a = load(q);
+    const auto b = load(q + step);
+    __builtin_amdgcn_sched_barrier(0);  // keep both elements' loads ahead of the uses
+    body(q, a);
+    body(q + step, b);
+  }
+  if (q < it.e1) body(q, load(q));
+}
+
 template <typename V>
 __global__ void kb_norm_b(BatchDev B, const vec<V>* __restrict__ b, double* part) {
   const BItem it = b_item(B);
@@ -182,8 +201,11 @@ __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V
   if (st->done) return;
   const c128 k1 = st->k1;
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
-  B_LOOP(q, it)
-    stv<NTS>(p + q, dn<V>(up(ldv<NTS>(v + q)) * up(ldv<false>(invd + q + io)) - up(ldv<NTS>(p + q)) * k1));
+  struct L {
+    vec<V> v, i, p;
+  };
+  b_loop2(it, [&](long long q) { return L{ldv<NTS>(v + q), ldv<false>(invd + q + io), ldv<NTS>(p + q)}; },
+          [&](long long q, const L& l) { stv<NTS>(p + q, dn<V>(up(l.v) * up(l.i) - up(l.p) * k1)); });
 }
 
 // batched stencil: one node per thread, work list of (system, x-plane,
@@ -259,7 +281,11 @@ __global__ void kb_vtnorm(BatchDev B, const vec<V>* __restrict__ t, const vec<V>
   if (st->done) return;
   const c128 beta = st->beta;
   double a[1] = {0.0};
-  B_LOOP(q, it) a[0] += abs2(up(dn<V>(up(ldv<NTS>(t + q)) - up(ldv<NTS>(v + q)) * beta)));
+  struct L {
+    vec<V> t, v;
+  };
+  b_loop2(it, [&](long long q) { return L{ldv<NTS>(t + q), ldv<NTS>(v + q)}; },
+          [&](long long, const L& l) { a[0] += abs2(up(dn<V>(up(l.t) - up(l.v) * beta))); });
   block_sum<1>(a);
   block_partial<1>(a, part);
 }
@@ -286,21 +312,25 @@ __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __re
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
   // x and r are stored on even iterations only, as in k_update
   const bool full = (st->iter & 1) == 0;
-  const vec<V> none{};
   double a[3] = {0.0, 0.0, 0.0};
-  if (full) {
-    B_LOOP(q, it)
-      update_elem<V, NTS, UPD_CATCHUP>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), ldv<NTS>(x + q),
-                                       ldv<NTS>(t + q), ldv<NTS>(s_ + q), ldv<NTS>(r + q),
-                                       ldv<NTS>(v + q), ldv<false>(invd + q + io), eta, beta, k2,
+  struct L {
+    vec<V> p, d, x, t, s, r, v, i;
+  };
+  auto load = [&](long long q, bool with_x) {
+    return L{ldv<NTS>(p + q), ldv<NTS>(d + q), with_x ? ldv<NTS>(x + q) : vec<V>{},
+             ldv<NTS>(t + q), ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
+             ldv<false>(invd + q + io)};
+  };
+  if (full)
+    b_loop2(it, [&](long long q) { return load(q, true); }, [&](long long q, const L& l) {
+      update_elem<V, NTS, UPD_CATCHUP>(q, l.p, l.d, l.x, l.t, l.s, l.r, l.v, l.i, eta, beta, k2,
                                        rho, d, x, s_, r, v, a);
-  } else {
-    B_LOOP(q, it)
-      update_elem<V, NTS, UPD_SKIP>(q, ldv<NTS>(p + q), ldv<NTS>(d + q), none, ldv<NTS>(t + q),
-                                    ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
-                                    ldv<false>(invd + q + io), eta, beta, k2, rho, d, x, s_, r,
-                                    v, a);
-  }
+    });
+  else
+    b_loop2(it, [&](long long q) { return load(q, false); }, [&](long long q, const L& l) {
+      update_elem<V, NTS, UPD_SKIP>(q, l.p, l.d, l.x, l.t, l.s, l.r, l.v, l.i, eta, beta, k2, rho,
+                                    d, x, s_, r, v, a);
+    });
   if ((int)blockIdx.x == B.bstart[it.s] && threadIdx.x == 0) st->lag = full ? 0 : 1;
   block_sum<3>(a);
   block_partial<3>(a, part);
