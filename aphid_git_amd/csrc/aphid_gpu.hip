@@ -1890,6 +1890,7 @@ struct aphid_batch {
   int giters32 = 0;
   long long *doff = nullptr, *dn4 = nullptr, *dopoff = nullptr;
   int *dop = nullptr, *dbstart = nullptr, *dtstart = nullptr;
+  int *dbsys = nullptr, *dtsys = nullptr;  // work item / tile -> system
   QmrState* st = nullptr;
   std::vector<QmrState> hst;
   double* part = nullptr;
@@ -1939,11 +1940,12 @@ void batch_free_device(aphid_batch* B) {
   for (auto& v : B->V)
     if (v) hipFree(v), v = nullptr;
   void* ps[] = {B->invd, B->dops, B->doff, B->dn4, B->dopoff, B->dop, B->dbstart, B->dtstart,
-                B->st, B->part, B->res, B->tst, B->dskip};
+                B->dbsys, B->dtsys, B->st, B->part, B->res, B->tst, B->dskip};
   for (void* p : ps)
     if (p) hipFree(p);
   B->invd = nullptr; B->dops = nullptr; B->doff = nullptr; B->dn4 = nullptr; B->dopoff = nullptr;
   B->dop = nullptr; B->dbstart = nullptr; B->dtstart = nullptr; B->st = nullptr; B->part = nullptr;
+  B->dbsys = nullptr; B->dtsys = nullptr;
   B->res = nullptr; B->tst = nullptr; B->dskip = nullptr;
   B->ntuned[0] = B->ntuned[1] = false;
   B->built = false;
@@ -1951,7 +1953,7 @@ void batch_free_device(aphid_batch* B) {
 
 BatchDev batch_dev(const aphid_batch* B) {
   return BatchDev{(int)B->sys.size(), B->doff, B->dn4, B->dop, B->dopoff, B->dbstart, B->dtstart,
-                  B->st};
+                  B->st, B->dbsys, B->dtsys};
 }
 
 // (re)build the flat device arrays after systems were added
@@ -1996,6 +1998,8 @@ int batch_build(aphid_batch* B) {
   CK(hipMalloc(&B->dop, sizeof(int) * ns));
   CK(hipMalloc(&B->dbstart, sizeof(int) * (ns + 1)));
   CK(hipMalloc(&B->dtstart, sizeof(int) * (ns + 1)));
+  CK(hipMalloc(&B->dbsys, sizeof(int) * (size_t)std::max(nb, 1)));
+  CK(hipMalloc(&B->dtsys, sizeof(int) * (size_t)std::max(nt, 1)));
   CK(hipMalloc(&B->st, sizeof(QmrState) * ns));
   CK(hipMalloc(&B->part, sizeof(double) * 3 * (size_t)std::max(nb, nt)));
   CK(hipMalloc(&B->res, sizeof(double) * 2 * ns));
@@ -2012,6 +2016,15 @@ int batch_build(aphid_batch* B) {
   CK(hipMemcpy(B->dop, op.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
   CK(hipMemcpy(B->dbstart, bstart.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
   CK(hipMemcpy(B->dtstart, tstart.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+  {
+    std::vector<int> bsys((size_t)nb), tsys((size_t)nt);
+    for (int q = 0; q < ns; ++q) {
+      for (int b = bstart[q]; b < bstart[q + 1]; ++b) bsys[b] = q;
+      for (int t = tstart[q]; t < tstart[q + 1]; ++t) tsys[t] = q;
+    }
+    if (nb) CK(hipMemcpy(B->dbsys, bsys.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    if (nt) CK(hipMemcpy(B->dtsys, tsys.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+  }
   CK(hipMemset(B->V[0], 0, sizeof(c128) * B->n4_tot));
   B->opoff = opoff;
   B->hst.assign(ns, QmrState{});

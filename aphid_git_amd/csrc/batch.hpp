@@ -31,17 +31,9 @@ struct BatchDev {
   const int* bstart;        // [nsys+1] streaming work-item prefix
   const int* tstart;        // [nsys+1] stencil tile prefix
   QmrState* st;             // [nsys]
+  const int* bsys;          // [bstart[nsys]] system of each streaming work item
+  const int* tsys;          // [tstart[nsys]] system of each stencil tile
 };
-
-__device__ __forceinline__ int find_sys(const int* __restrict__ start, int nsys, int b) {
-  int lo = 0, hi = nsys;  // start[lo] <= b < start[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (start[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // streaming work item of this block: system, global element range [e0, e1)
 struct BItem {
@@ -50,7 +42,7 @@ struct BItem {
 };
 __device__ __forceinline__ BItem b_item(const BatchDev& B) {
   const int b = blockIdx.x;
-  const int s = find_sys(B.bstart, B.nsys, b);
+  const int s = B.bsys[b];
   const long long lb = b - B.bstart[s];
   const long long e0 = lb * kBElems, e1 = min(e0 + kBElems, B.n4[s]);
   return {s, B.off[s] + e0, B.off[s] + e1};
@@ -218,7 +210,7 @@ __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __re
                                                double* part) {
   using C = cplx<R>;
   const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
-  const int s = find_sys(B.tstart, B.nsys, b);
+  const int s = B.tsys[b];
   double eacc_re = 0.0, eacc_im = 0.0;
   const bool live = !(with_eps && B.st[s].done);
   if (live) {
