@@ -223,16 +223,16 @@ __global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
 
 // one element of step D: d, x, s, r, v updates; accumulates ||r||^2 and
 // v^T M^-1 v (src/aphid_qmr.f90:347-369, 306-311).
-// UPD_EAGER stores x and r every iteration.  The single-system solve stores
-// them every second iteration instead: an odd iteration (UPD_SKIP) leaves
-// them one step behind (x_{k-1}, r_{k-1} stay in memory; r_k is formed for
-// its norm only), and the even one after it (UPD_CATCHUP) first rebuilds
-// x_{k-1} = x_{k-2} + d_{k-1} and r_{k-1} = r_{k-2} - s_{k-1} in registers
-// from the d and s it reads anyway, with the operations storing them would
-// have taken, so every value is bit-for-bit the one of the eager
-// recurrence.  A solve that stops after an odd iteration is completed by
-// k_catch_up.
-enum UpdMode { UPD_EAGER, UPD_SKIP, UPD_CATCHUP };
+// x and r are stored every second iteration: an odd iteration of a solve
+// (UPD_SKIP) leaves them one step behind (x_{k-1}, r_{k-1} stay in memory;
+// r_k is formed for its norm only), and the even one after it
+// (UPD_CATCHUP) first rebuilds x_{k-1} = x_{k-2} + d_{k-1} and
+// r_{k-1} = r_{k-2} - s_{k-1} in registers from the d and s it reads
+// anyway, with the operations storing them would have taken, so every
+// value is bit-for-bit the one of storing x and r each time.  A solve that
+// stops after an odd iteration is completed by k_catch_up (kb_catch_up in
+// a batch).
+enum UpdMode { UPD_SKIP, UPD_CATCHUP };
 template <typename V, bool NT, int MODE>
 __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_,
                                             vec<V> sq, vec<V> rq, vec<V> vq, vec<V> iq, c128 eta,
@@ -243,7 +243,6 @@ __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, v
   const c128 tq = up(tq_);
   const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
   stv<NT>(d + q, ddv);
-  if (MODE == UPD_EAGER) stv<NT>(x + q, dn<V>(up(xq) + up(ddv)));
   if (MODE == UPD_CATCHUP) stv<NT>(x + q, dn<V>(up(dn<V>(up(xq) + up(dq))) + up(ddv)));
   const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
   stv<NT>(s + q, ssv);
