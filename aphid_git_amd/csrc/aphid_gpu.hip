@@ -1896,6 +1896,12 @@ struct aphid_batch {
   double* part = nullptr;
   double* res = nullptr;
   int nstream = 0, ntile = 0;
+  // batched stencil: the x-march over flat chunks (xch planes per block, an
+  // LDS window of wstride nodes per array) unless a system's z-rows are too
+  // long for the window (or APHID_BATCH_DIRECT=1), then the
+  // one-node-per-thread kernel
+  bool march = true;
+  int xch = 8, wstride = 0;
   bool built = false;
   bool solved = false;
   hipGraphExec_t gexec = nullptr;
@@ -1971,6 +1977,11 @@ int batch_build(aphid_batch* B) {
   B->op4_tot = o;
   o = 0;
   int nb = 0, nt = 0;
+  int nzmax = 0;
+  for (int q = 0; q < ns; ++q) nzmax = std::max(nzmax, B->ops[B->sys[q].op]->nz);
+  B->wstride = kBT + 2 * nzmax;
+  B->march = 7 * (size_t)B->wstride * sizeof(c128) <= 65536;
+  if (const char* e = getenv("APHID_BATCH_DIRECT")) B->march = B->march && e[0] != '1';
   for (int q = 0; q < ns; ++q) {
     const aphid_ctx* c = B->ops[B->sys[q].op];
     off[q] = o;
@@ -1982,7 +1993,8 @@ int batch_build(aphid_batch* B) {
     bstart[q] = nb;
     nb += (int)((n4[q] + kBElems - 1) / kBElems);
     tstart[q] = nt;
-    nt += c->nx * (int)((c->ny * c->nz + kBT - 1) / kBT);
+    const int nch = (int)((c->ny * c->nz + kBT - 1) / kBT);
+    nt += nch * (B->march ? (c->nx + B->xch - 1) / B->xch : c->nx);
   }
   bstart[ns] = nb;
   tstart[ns] = nt;
@@ -2035,6 +2047,20 @@ int batch_build(aphid_batch* B) {
 // kernel k (0 p-update, 1 stencil with eps, 2 ||t - beta v||, 3 fused
 // update) of a batch iteration, each with its finalize, with the
 // non-temporal hints nts
+// the batched stencil on every system: t = K u, with the eps partials when
+// with_eps (x-march over flat chunks, or one node per thread, see batch_build)
+template <typename V>
+void batch_apply(aphid_batch* B, const BatchDev& D, const OpArgs<V>* ops, const cplx<V>* u,
+                 cplx<V>* t, int with_eps) {
+  if (B->march)
+    hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT),
+                       7 * (size_t)B->wstride * sizeof(cplx<V>), B->s, D, ops, u, t, with_eps,
+                       B->part, B->xch, B->wstride);
+  else
+    hipLaunchKernelGGL((kb_apply_direct<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, u,
+                       t, with_eps, B->part);
+}
+
 template <typename V>
 void batch_launch(aphid_batch* B, const BatchDev& D, int k, const int* nts,
                   const OpArgs<V>* ops, cplx<V>* x, cplx<V>* v, cplx<V>* p, cplx<V>* t, cplx<V>* d,
@@ -2049,8 +2075,7 @@ void batch_launch(aphid_batch* B, const BatchDev& D, int k, const int* nts,
         hipLaunchKernelGGL((kb_pupdate<V, false>), gs, bs, 0, B->s, D, v, invd, p);
       break;
     case 1:
-      hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, p, t, 1,
-                         B->part);
+      batch_apply<V>(B, D, ops, p, t, 1);
       hipLaunchKernelGGL(kb_fin_eps, dim3(ns), bs, 0, B->s, D, B->part);
       break;
     case 2:
@@ -2329,9 +2354,7 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
     hipLaunchKernelGGL(kb_fin_norm_b_tol, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, itol);
   else
     hipLaunchKernelGGL(kb_fin_norm_b, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, tol);
-  if (use_x0)
-    hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, x, t, 0,
-                       B->part);
+  if (use_x0) batch_apply<V>(B, D, ops, x, t, 0);
   hipLaunchKernelGGL(kb_init_r<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, b,
                      use_x0 ? (const cplx<V>*)t : nullptr, r, B->part);
   hipLaunchKernelGGL(kb_fin_init_r, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
@@ -2415,8 +2438,7 @@ int batch_solve_single(aphid_batch* B, int q, double tol, int max_it, bool use_x
 int batch_true_residual(aphid_batch* B) {
   const BatchDev D = batch_dev(B);
   const int ns = (int)B->sys.size();
-  hipLaunchKernelGGL((kb_apply<double, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, B->dops,
-                     B->V[0], B->V[8], 0, B->part);
+  batch_apply<double>(B, D, B->dops, B->V[0], B->V[8], 0);
   hipLaunchKernelGGL(kb_resnorm<double>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, B->V[1],
                      B->V[8], B->part);
   hipLaunchKernelGGL(kb_fin_resnorm, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, B->res);

@@ -200,14 +200,160 @@ __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V
           [&](long long q, const L& l) { stv<NTS>(p + q, dn<V>(up(l.v) * up(l.i) - up(l.p) * k1)); });
 }
 
-// batched stencil: one node per thread, work list of (system, x-plane,
-// flat in-plane chunk of NT nodes) over all systems (flat in-plane chunks fit
-// any (ny, nz) without idle lanes); eps partial per work item
+// batched stencil, x-march over flat in-plane chunks.  A block owns NT
+// consecutive nodes of one system's (j, k) plane (flat index q = j*nz + k,
+// which fits any (ny, nz) without idle lanes) and xch consecutive x-planes:
+//   - the in-plane neighbours (q +- 1, q +- nz) come from an LDS window of
+//     the plane holding the chunk plus nz nodes on each side (stride
+//     wstride >= NT + 2 nz for every system of the batch); a halo node
+//     holds the four fields and the k^2 grids its side needs (k2y and k2z
+//     on the low side, k2p on the high side);
+//   - the x-neighbours (i +- 1) of the own node live in a register queue;
+//   - a neighbour outside the domain is read where rows4 multiplies it by
+//     an exact zero (the integer-division wall masks of the coefficient
+//     tables) or not at all, so a flat index that wraps to the next z-row
+//     needs no test; the window's nodes outside the plane hold zeros.
+// eps partial per block.
+template <typename R, int NT>
+struct FlatAcc {
+  using C = cplx<R>;
+  const C* sm;  // 7 arrays of stride ws: Ax Ay Az Phi k2y k2z k2p
+  int ws, w, nz;
+  const C* pa;
+  const C* ca;
+  const C* na;
+  C k2x_p, k2x_c, k2p_c, k2p_n;
+  __device__ __forceinline__ C at(int a, int o) const { return sm[a * ws + w + o]; }
+  __device__ __forceinline__ C fv(int c, int d) const {
+    switch (d) {
+      case 0: return ca[c];
+      case 1: return na[c];
+      case 2: return pa[c];
+      case 3: return at(c, nz);
+      case 4: return at(c, -nz);
+      case 5: return at(c, 1);
+      default: return at(c, -1);
+    }
+  }
+  __device__ __forceinline__ C kv(int c, int d) const {
+    if (c == 0) return d == 2 ? k2x_p : k2x_c;
+    if (c == 3) {
+      if (d == 0) return k2p_c;
+      if (d == 1) return k2p_n;
+      return d == 3 ? at(6, nz) : at(6, 1);
+    }
+    if (c == 1) return d == 4 ? at(4, -nz) : at(4, 0);
+    return d == 6 ? at(5, -1) : at(5, 0);
+  }
+};
+
 template <typename R, int NT>
 __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __restrict__ ops,
                                                const cplx<R>* __restrict__ in,
                                                cplx<R>* __restrict__ out, int with_eps,
-                                               double* part) {
+                                               double* part, int xch, int wstride) {
+  using C = cplx<R>;
+  extern __shared__ __align__(16) unsigned char kb_smem[];
+  C* sm = reinterpret_cast<C*>(kb_smem);
+  constexpr unsigned CS = sizeof(C);
+  const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
+  const int s = B.tsys[b];
+  double eacc_re = 0.0, eacc_im = 0.0;
+  const bool live = !(with_eps && B.st[s].done);
+  if (live) {
+    const OpArgs<R>& op = ops[B.op[s]];
+    const int nx = op.nx, ny = op.ny, nz = op.nz;
+    const int pl = ny * nz, nch = (pl + NT - 1) / NT;
+    const int t = b - B.tstart[s];
+    const int ch = t % nch, xc = t / nch;
+    const int q0 = ch * NT, q = q0 + (int)threadIdx.x;
+    const bool own = q < pl;
+    const int j = own ? q / nz : 0, k = own ? q - j * nz : 0;
+    const int i0 = xc * xch, i1 = min(nx, i0 + xch);
+    const unsigned nnb = (unsigned)op.nn * CS, plb = (unsigned)pl * CS;
+    const unsigned colb = own ? (unsigned)q * CS : 0u;
+    const C* __restrict__ sin = in + B.off[s];
+    const C* __restrict__ K2 = op.k2;
+    C* __restrict__ sout = out + B.off[s];
+    const int ws = wstride, w = nz + (int)threadIdx.x;
+    auto ldc = [&](const C* __restrict__ base, int c, int i) -> C {
+      return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb) : C{R(0), R(0)};
+    };
+    C pa[4], ca[4], na[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      pa[c] = ldc(sin, c, i0 - 1);
+      ca[c] = ldc(sin, c, i0);
+    }
+    C k2x_p = ldc(K2, 0, i0 - 1), k2x_c = ldc(K2, 0, i0), k2p_c = ldc(K2, 3, i0);
+    for (int i = i0; i < i1; ++i) {
+      // stage plane i: own values, then the window's halo (nz nodes below
+      // the chunk: fields, k2y, k2z; nz above: fields, k2p)
+      sm[4 * ws + w] = ldc(K2, 1, i);
+      sm[5 * ws + w] = ldc(K2, 2, i);
+      for (int h = threadIdx.x; h < 2 * nz; h += NT) {
+        const bool lo = h < nz;
+        const int hq = lo ? q0 - nz + h : q0 + NT + (h - nz);
+        const int hw = lo ? h : NT + h;
+        const bool hin = hq >= 0 && hq < pl;
+        const unsigned g = (unsigned)i * plb + (hin ? (unsigned)hq * CS : 0u);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sm[c * ws + hw] = hin ? ldo(sin, c * nnb + g) : C{R(0), R(0)};
+        if (lo) {
+          sm[4 * ws + hw] = hin ? ldo(K2, nnb + g) : C{R(0), R(0)};
+          sm[5 * ws + hw] = hin ? ldo(K2, 2 * nnb + g) : C{R(0), R(0)};
+        } else {
+          sm[6 * ws + hw] = hin ? ldo(K2, 3 * nnb + g) : C{R(0), R(0)};
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm[c * ws + w] = ca[c];
+      sm[6 * ws + w] = k2p_c;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) na[c] = ldc(sin, c, i + 1);
+      const C k2x_n = ldc(K2, 0, i + 1), k2p_n = ldc(K2, 3, i + 1);
+      __syncthreads();
+      if (own) {
+        const unsigned ob = (unsigned)i * plb + colb;
+        FlatAcc<R, NT> a{sm, ws, w, nz, pa, ca, na, k2x_p, k2x_c, k2p_c, k2p_n};
+        C rx, ry, rz, rp;
+        rows4<R>(i, j, k, nx, ny, nz, axis_at(op, 0, i), axis_at(op, 1, j), axis_at(op, 2, k), a,
+                 rx, ry, rz, rp);
+        sto(sout, ob, rx);
+        sto(sout, nnb + ob, ry);
+        sto(sout, 2 * nnb + ob, rz);
+        sto(sout, 3 * nnb + ob, rp);
+        if (with_eps) eps_acc<R>(ca, rx, ry, rz, rp, eacc_re, eacc_im);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        pa[c] = ca[c];
+        ca[c] = na[c];
+      }
+      k2x_p = k2x_c;
+      k2x_c = k2x_n;
+      k2p_c = k2p_n;
+    }
+  }
+  if (with_eps) {
+    double v[2] = {eacc_re, eacc_im};
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+      part[(size_t)b * 2] = v[0];
+      part[(size_t)b * 2 + 1] = v[1];
+    }
+  }
+}
+
+// batched stencil, one node per thread: work list of (system, x-plane,
+// flat in-plane chunk of NT nodes) over all systems; eps partial per work
+// item.  Used when the march's LDS window would not fit (very long z-rows).
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void kb_apply_direct(BatchDev B, const OpArgs<R>* __restrict__ ops,
+                                                      const cplx<R>* __restrict__ in,
+                                                      cplx<R>* __restrict__ out, int with_eps,
+                                                      double* part) {
   using C = cplx<R>;
   const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
   const int s = B.tsys[b];

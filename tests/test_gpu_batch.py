@@ -91,6 +91,38 @@ def test_batched_iterate_after_odd_and_even_iteration_counts(gpu_mod, oracle, ma
         assert np.linalg.norm(flat4(B.solution(0)) - xo) / np.linalg.norm(xo) < 1e-11
 
 
+def test_both_batched_stencils_match_the_oracle(gpu_mod, oracle, monkeypatch):
+    """The x-march over flat chunks (default) and the one-node-per-thread
+    batched stencil (APHID_BATCH_DIRECT=1, the fallback for long z-rows)
+    solve the same uneven pack to the oracle's fields."""
+    cases = _uneven_cases()
+    sols = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("APHID_BATCH_DIRECT", mode)
+        with gpu_mod.BatchSolver() as B:
+            single = gpu_mod.Solver()
+            rhs = []
+            for o, (c, srcs) in enumerate(cases):
+                B.add_case_operator(c)  # every operator before the first system
+                single.set_mesh(c.xn, c.yn, c.zn)
+                rhs += [(o, single.build_rhs(5, [src])[0]) for src in srcs]
+            single.close()
+            for o, b in rhs:
+                B.add_system(o, b)
+            k = len(rhs)
+            reps, _ = B.solve(1e-9, 5000)
+            assert all(r["status"] == 0 and r["true_rel_res"] < 1.05e-9 for r in reps)
+            sols[mode] = [flat4(B.solution(q)) for q in range(k)]
+    q = 0
+    for c, srcs in cases:
+        for src in srcs:
+            S = oracle.System(meshgen.Case(c.xn, c.yn, c.zn, c.model, c.cells, c.freq, 5, [src]))
+            xo = flat4(S.solve(tol=1e-9, max_it=5000)[0])
+            for mode in ("0", "1"):
+                assert np.linalg.norm(sols[mode][q] - xo) / np.linalg.norm(xo) < 1e-7
+            q += 1
+
+
 def test_shared_operators_frequency_sweep_and_warm_restart(gpu_mod):
     """config-1 family: 3 frequencies x 3 dipoles (+X, +Y, +Z at the surface
     centre) share one operator per frequency; then a warm restart from the
