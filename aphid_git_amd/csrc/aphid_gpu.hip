@@ -209,21 +209,21 @@ namespace {
 
 void free_mesh(aphid_ctx* c) {
   for (int p = 0; p < 2; ++p) {
-    if (c->gexec[p]) hipGraphExecDestroy(c->gexec[p]);
+    if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
     c->gexec[p] = nullptr;
     c->giters[p] = 0;
   }
   void* ptrs[] = {c->tab64, c->tab32, c->dmet, c->sig, c->k2, c->k2f, c->diag, c->invd, c->invdf,
                   c->stage, c->stagef, c->hist};
   for (void* p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   for (auto& v : c->V) {
-    if (v && !c->vborrowed) hipFree(v);
+    if (v && !c->vborrowed) (void)hipFree(v);
     v = nullptr;
   }
   c->vborrowed = false;
   for (auto& v : c->F)
-    if (v) hipFree(v), v = nullptr;
+    if (v) (void)hipFree(v), v = nullptr;
   c->tab64 = nullptr; c->tab32 = nullptr; c->dmet = nullptr; c->sig = nullptr; c->k2 = nullptr;
   c->k2f = nullptr; c->diag = nullptr; c->invd = nullptr; c->invdf = nullptr; c->stage = nullptr;
   c->stagef = nullptr; c->hist = nullptr; c->hist_cap = 0;
@@ -1036,17 +1036,17 @@ int aphid_ctx_create(aphid_ctx** out, int device) {
 
 int aphid_ctx_destroy(aphid_ctx* c) {
   if (!c) return 0;
-  hipSetDevice(c->dev);
-  if (c->s) hipStreamSynchronize(c->s);
+  (void)hipSetDevice(c->dev);
+  if (c->s) (void)hipStreamSynchronize(c->s);
   free_mesh(c);
-  if (c->st) hipFree(c->st);
-  if (c->tst) hipFree(c->tst);
-  if (c->hst) hipHostFree(c->hst);
-  if (c->part) hipFree(c->part);
-  if (c->resbuf) hipFree(c->resbuf);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->s) hipStreamDestroy(c->s);
+  if (c->st) (void)hipFree(c->st);
+  if (c->tst) (void)hipFree(c->tst);
+  if (c->hst) (void)hipHostFree(c->hst);
+  if (c->part) (void)hipFree(c->part);
+  if (c->resbuf) (void)hipFree(c->resbuf);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return 0;
 }
@@ -1451,7 +1451,7 @@ int aphid_solve_resident(aphid_ctx* c, double tol, int max_it, unsigned flags,
     CK(hipMalloc(&c->hist, sizeof(double) * cap));
     c->hist_cap = cap;
     for (auto& g : c->gexec)
-      if (g) hipGraphExecDestroy(g), g = nullptr;  // graphs captured the old pointer
+      if (g) (void)hipGraphExecDestroy(g), g = nullptr;  // graphs captured the old pointer
   }
   if (!have_x0) CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * 4 * c->nn, c->s));
   double secs = 0.0;
@@ -1734,7 +1734,7 @@ int aphid_set_tile(aphid_ctx* c, int precision, int tk, int tj, int chunk) {
     return fail(APHID_E_ARG, "too many blocks for this tile");
   c->tile[precision] = {tk, tj, chunk};
   c->tuned[precision] = true;
-  if (c->gexec[precision]) hipGraphExecDestroy(c->gexec[precision]), c->gexec[precision] = nullptr;
+  if (c->gexec[precision]) (void)hipGraphExecDestroy(c->gexec[precision]), c->gexec[precision] = nullptr;
   return 0;
 }
 
@@ -1831,7 +1831,7 @@ int aphid_time_copy(aphid_ctx* c, double bytes, int reps, double* seconds) {
   CK(hipMalloc(&a, sizeof(c128) * n));
   hipError_t e = hipMalloc(&b, sizeof(c128) * n);
   if (e != hipSuccess) {
-    hipFree(a);
+    (void)hipFree(a);
     return fail(APHID_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   int rc = 0;
@@ -1846,8 +1846,8 @@ int aphid_time_copy(aphid_ctx* c, double bytes, int reps, double* seconds) {
     if ((e = hipEventSynchronize(c->ev1)) != hipSuccess) break;
     e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
   } while (0);
-  hipFree(a);
-  hipFree(b);
+  (void)hipFree(a);
+  (void)hipFree(b);
   if (e != hipSuccess) rc = fail(APHID_E_HIP, std::string("copy probe: ") + hipGetErrorString(e));
   *seconds = 1e-3 * ms / reps;
   return rc;
@@ -1927,28 +1927,28 @@ void batch_free_device(aphid_batch* B) {
   for (aphid_ctx* c : B->ops) {
     if (!c || !c->vborrowed) continue;
     for (int p = 0; p < 2; ++p) {
-      if (c->gexec[p]) hipGraphExecDestroy(c->gexec[p]);
+      if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
       c->gexec[p] = nullptr;
       c->gvec[p] = nullptr;
     }
     for (auto& v : c->V) v = nullptr;
     c->vborrowed = false;
   }
-  if (B->gexec) hipGraphExecDestroy(B->gexec), B->gexec = nullptr;
-  if (B->gexec32) hipGraphExecDestroy(B->gexec32), B->gexec32 = nullptr;
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec), B->gexec = nullptr;
+  if (B->gexec32) (void)hipGraphExecDestroy(B->gexec32), B->gexec32 = nullptr;
   for (auto& v : B->F)
-    if (v) hipFree(v), v = nullptr;
+    if (v) (void)hipFree(v), v = nullptr;
   void* ps32[] = {B->invdf, B->dops32, B->scale, B->itol, B->nopen, B->budget};
   for (void* p : ps32)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   B->invdf = nullptr; B->dops32 = nullptr; B->scale = nullptr; B->itol = nullptr; B->nopen = nullptr;
   B->budget = nullptr;
   for (auto& v : B->V)
-    if (v) hipFree(v), v = nullptr;
+    if (v) (void)hipFree(v), v = nullptr;
   void* ps[] = {B->invd, B->dops, B->doff, B->dn4, B->dopoff, B->dop, B->dbstart, B->dtstart,
                 B->dbsys, B->dtsys, B->st, B->part, B->res, B->tst, B->dskip};
   for (void* p : ps)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   B->invd = nullptr; B->dops = nullptr; B->doff = nullptr; B->dn4 = nullptr; B->dopoff = nullptr;
   B->dop = nullptr; B->dbstart = nullptr; B->dtstart = nullptr; B->st = nullptr; B->part = nullptr;
   B->dbsys = nullptr; B->dtsys = nullptr;
@@ -2236,16 +2236,16 @@ int aphid_batch_create(aphid_batch** out, int device) {
 
 int aphid_batch_destroy(aphid_batch* B) {
   if (!B) return 0;
-  hipSetDevice(B->dev);
-  if (B->s) hipStreamSynchronize(B->s);
+  (void)hipSetDevice(B->dev);
+  if (B->s) (void)hipStreamSynchronize(B->s);
   batch_free_device(B);
-  if (B->stage) hipFree(B->stage);
+  if (B->stage) (void)hipFree(B->stage);
   for (auto& q : B->sys)
-    if (q.staged) hipFree(q.staged);
+    if (q.staged) (void)hipFree(q.staged);
   for (aphid_ctx* c : B->ops) aphid_ctx_destroy(c);
-  if (B->ev0) hipEventDestroy(B->ev0);
-  if (B->ev1) hipEventDestroy(B->ev1);
-  if (B->s) hipStreamDestroy(B->s);
+  if (B->ev0) (void)hipEventDestroy(B->ev0);
+  if (B->ev1) (void)hipEventDestroy(B->ev1);
+  if (B->s) (void)hipStreamDestroy(B->s);
   delete B;
   return 0;
 }
@@ -2295,7 +2295,7 @@ int aphid_batch_add_system(aphid_batch* B, int op, const double* bx, const doubl
     if (e != hipSuccess) rc = fail(APHID_E_HIP, hipGetErrorString(e));
   }
   if (rc) {
-    hipFree(tmp);
+    (void)hipFree(tmp);
     B->sys.pop_back();
     return rc;
   }
@@ -2371,7 +2371,7 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
   hipGraphExec_t& gx = f64 ? B->gexec : B->gexec32;
   int& gk = f64 ? B->giters : B->giters32;
   if (!gx || gk != k) {
-    if (gx) hipGraphExecDestroy(gx), gx = nullptr;
+    if (gx) (void)hipGraphExecDestroy(gx), gx = nullptr;
     if (use_graphs()) {
       hipGraph_t g;
       CK(hipStreamBeginCapture(B->s, hipStreamCaptureModeThreadLocal));
@@ -2423,7 +2423,7 @@ int batch_solve_single(aphid_batch* B, int q, double tol, int max_it, bool use_x
   aphid_ctx* c = B->ops[sy.op];
   if (!c->vborrowed)
     for (auto& v : c->V)
-      if (v) hipFree(v), v = nullptr;
+      if (v) (void)hipFree(v), v = nullptr;
   for (int i = 0; i < 9; ++i) c->V[i] = B->V[i] + sy.off;
   c->vborrowed = true;
   int rc = autotune(c, 0);

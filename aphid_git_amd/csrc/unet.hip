@@ -522,8 +522,8 @@ aphid_unet* aphid_unet_create(int device, int nx, int ny, int nz, unsigned long 
 
 void aphid_unet_destroy(aphid_unet* n) {
   if (!n) return;
-  hipSetDevice(n->dev);
-  hipStreamSynchronize(n->s);
+  (void)hipSetDevice(n->dev);
+  (void)hipStreamSynchronize(n->s);
   free_net(n);
   delete n;
 }
@@ -611,7 +611,7 @@ int aphid_unet_reserve(aphid_unet* n, int cap) {
   hipError_t e = hipMalloc(&n->ds_in, sizeof(bf16_t) * per * (size_t)std::max(cap, 1));
   if (e == hipSuccess) e = hipMalloc(&n->ds_tg, sizeof(bf16_t) * per * (size_t)std::max(cap, 1));
   if (e != hipSuccess) {
-    if (n->ds_in) hipFree(n->ds_in);
+    if (n->ds_in) (void)hipFree(n->ds_in);
     n->ds_in = nullptr;
     return aphid_fail(APHID_E_NOMEM, "aphid_unet_reserve: dataset does not fit in device memory");
   }
