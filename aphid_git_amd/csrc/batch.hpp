@@ -20,7 +20,7 @@
 #include "qmr.hpp"
 #include "stencil.hpp"
 
-constexpr int kBElems = 4096;  // elements per streaming work item (256 threads x 16)
+constexpr int kBElems = 512;  // elements per streaming work item (256 threads x 2)
 
 struct BatchDev {
   int nsys;
