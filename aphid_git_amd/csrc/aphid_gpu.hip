@@ -2367,7 +2367,9 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
   if (rc) return rc;
   if (batch_all_done(B)) return 0;
   const double bytes = 22.0 * (f64 ? 16.0 : 8.0) * (double)B->n4_tot;
-  const int k = std::max(4, std::min(128, (int)std::ceil(1e-2 / std::max(bytes / 4.0e12, 25e-6))));
+  // about 30 ms of work per replay (the host reads every system's state
+  // after each one); iterations past a system's stop return at once
+  const int k = std::max(4, std::min(128, (int)std::ceil(3e-2 / std::max(bytes / 4.0e12, 25e-6))));
   hipGraphExec_t& gx = f64 ? B->gexec : B->gexec32;
   int& gk = f64 ? B->giters : B->giters32;
   if (!gx || gk != k) {
