@@ -1919,7 +1919,7 @@ struct aphid_batch {
 
 namespace {
 
-constexpr int kBT = 256;  // batched stencil block (flat in-plane chunk)
+constexpr int kBT = 128;  // batched stencil block (flat in-plane chunk of 128 nodes)
 
 void batch_free_device(aphid_batch* B) {
   // operator contexts that borrowed slices of the flat vectors let go of
