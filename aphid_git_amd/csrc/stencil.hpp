@@ -249,7 +249,6 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   const bool own = (j < ny) && (k < nz);
   const unsigned colb = own ? ((unsigned)j * nz + k) * CS : 0u;
   const unsigned plb = pl * CS;
-  const C zero = {R(0), R(0)};
 
   if (threadIdx.x < TJ) {
     const int jj = bj * TJ + threadIdx.x;
@@ -272,7 +271,8 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
 
   // own value of component c (0..3) or k2 grid c (0..3) at plane i
   auto ldc = [&](const C* __restrict__ base, int c, int i) -> C {
-    return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb) : zero;
+    return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb)
+                                     : C{R(0), R(0)};
   };
 
   // halo cell of this thread (if any): rows j=-1 and j=TJ over the tile's
@@ -315,8 +315,8 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     if (hv) {
       const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : zero;
-      sm[3 + hc][hl] = hin ? ldo(K2, hc * nnb + g) : zero;
+      for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : C{R(0), R(0)};
+      sm[3 + hc][hl] = hin ? ldo(K2, hc * nnb + g) : C{R(0), R(0)};
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
