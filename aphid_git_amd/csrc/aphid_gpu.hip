@@ -1920,6 +1920,10 @@ struct aphid_batch {
 namespace {
 
 constexpr int kBT = 128;  // batched stencil block (flat in-plane chunk of 128 nodes)
+// LDS window strides of the batched x-march, by the longest z-row of the
+// batch (nz <= 40, 72, 136, 228); the last keeps 7 arrays of complex128 in
+// 64 KB
+constexpr int kBWindow[4] = {kBT + 2 * 40, kBT + 2 * 72, kBT + 2 * 136, kBT + 2 * 228};
 
 void batch_free_device(aphid_batch* B) {
   // operator contexts that borrowed slices of the flat vectors let go of
@@ -1979,8 +1983,10 @@ int batch_build(aphid_batch* B) {
   int nb = 0, nt = 0;
   int nzmax = 0;
   for (int q = 0; q < ns; ++q) nzmax = std::max(nzmax, B->ops[B->sys[q].op]->nz);
-  B->wstride = kBT + 2 * nzmax;
-  B->march = 7 * (size_t)B->wstride * sizeof(c128) <= 65536;
+  B->wstride = 0;
+  for (int w : kBWindow)
+    if (!B->wstride && w >= kBT + 2 * nzmax) B->wstride = w;
+  B->march = B->wstride > 0;
   if (const char* e = getenv("APHID_BATCH_DIRECT")) B->march = B->march && e[0] != '1';
   for (int q = 0; q < ns; ++q) {
     const aphid_ctx* c = B->ops[B->sys[q].op];
@@ -2052,11 +2058,19 @@ int batch_build(aphid_batch* B) {
 template <typename V>
 void batch_apply(aphid_batch* B, const BatchDev& D, const OpArgs<V>* ops, const cplx<V>* u,
                  cplx<V>* t, int with_eps) {
-  if (B->march)
-    hipLaunchKernelGGL((kb_apply<V, kBT>), dim3(B->ntile), dim3(kBT),
-                       7 * (size_t)B->wstride * sizeof(cplx<V>), B->s, D, ops, u, t, with_eps,
-                       B->part, B->xch, B->wstride);
-  else
+  if (B->march) {
+    const size_t lds = 7 * (size_t)B->wstride * sizeof(cplx<V>);
+#define APHID_KB_APPLY(WS)                                                                    \
+  hipLaunchKernelGGL((kb_apply<V, kBT, WS>), dim3(B->ntile), dim3(kBT), lds, B->s, D, ops, u, t, \
+                     with_eps, B->part, B->xch)
+    switch (B->wstride) {
+      case kBWindow[0]: APHID_KB_APPLY(kBWindow[0]); break;
+      case kBWindow[1]: APHID_KB_APPLY(kBWindow[1]); break;
+      case kBWindow[2]: APHID_KB_APPLY(kBWindow[2]); break;
+      default: APHID_KB_APPLY(kBWindow[3]);
+    }
+#undef APHID_KB_APPLY
+  } else
     hipLaunchKernelGGL((kb_apply_direct<V, kBT>), dim3(B->ntile), dim3(kBT), 0, B->s, D, ops, u,
                        t, with_eps, B->part);
 }

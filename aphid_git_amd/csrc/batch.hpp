@@ -205,7 +205,7 @@ __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V
 // which fits any (ny, nz) without idle lanes) and xch consecutive x-planes:
 //   - the in-plane neighbours (q +- 1, q +- nz) come from an LDS window of
 //     the plane holding the chunk plus nz nodes on each side (stride
-//     wstride >= NT + 2 nz for every system of the batch); a halo node
+//     WS >= NT + 2 nz for every system of the batch); a halo node
 //     holds the four fields and the k^2 grids its side needs (k2y and k2z
 //     on the low side, k2p on the high side);
 //   - the x-neighbours (i +- 1) of the own node live in a register queue;
@@ -214,11 +214,12 @@ __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V
 //     tables) or not at all, so a flat index that wraps to the next z-row
 //     needs no test; the window's nodes outside the plane hold zeros.
 // eps partial per block.
-template <typename R, int NT>
+template <typename R, int NT, int WS>
 struct FlatAcc {
   using C = cplx<R>;
+  static constexpr int ws = WS;
   const C* sm;  // 7 arrays of stride ws: Ax Ay Az Phi k2y k2z k2p
-  int ws, w, nz;
+  int w, nz;
   const C* pa;
   const C* ca;
   const C* na;
@@ -247,11 +248,11 @@ struct FlatAcc {
   }
 };
 
-template <typename R, int NT>
+template <typename R, int NT, int WS>
 __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __restrict__ ops,
                                                const cplx<R>* __restrict__ in,
                                                cplx<R>* __restrict__ out, int with_eps,
-                                               double* part, int xch, int wstride) {
+                                               double* part, int xch) {
   using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char kb_smem[];
   C* sm = reinterpret_cast<C*>(kb_smem);
@@ -275,9 +276,10 @@ __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __re
     const C* __restrict__ sin = in + B.off[s];
     const C* __restrict__ K2 = op.k2;
     C* __restrict__ sout = out + B.off[s];
-    const int ws = wstride, w = nz + (int)threadIdx.x;
+    constexpr int ws = WS;  // compile-time, so window offsets fold into the LDS immediates
+    const int w = nz + (int)threadIdx.x;
     auto ldc = [&](const C* __restrict__ base, int c, int i) -> C {
-      return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb) : C{R(0), R(0)};
+      return (own && i >= 0 && i < nx) ? ldg(base, c * nnb + (unsigned)i * plb + colb) : C{R(0), R(0)};
     };
     C pa[4], ca[4], na[4];
 #pragma unroll
@@ -298,12 +300,12 @@ __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __re
         const bool hin = hq >= 0 && hq < pl;
         const unsigned g = (unsigned)i * plb + (hin ? (unsigned)hq * CS : 0u);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sm[c * ws + hw] = hin ? ldo(sin, c * nnb + g) : C{R(0), R(0)};
+        for (int c = 0; c < 4; ++c) sm[c * ws + hw] = hin ? ldg(sin, c * nnb + g) : C{R(0), R(0)};
         if (lo) {
-          sm[4 * ws + hw] = hin ? ldo(K2, nnb + g) : C{R(0), R(0)};
-          sm[5 * ws + hw] = hin ? ldo(K2, 2 * nnb + g) : C{R(0), R(0)};
+          sm[4 * ws + hw] = hin ? ldg(K2, nnb + g) : C{R(0), R(0)};
+          sm[5 * ws + hw] = hin ? ldg(K2, 2 * nnb + g) : C{R(0), R(0)};
         } else {
-          sm[6 * ws + hw] = hin ? ldo(K2, 3 * nnb + g) : C{R(0), R(0)};
+          sm[6 * ws + hw] = hin ? ldg(K2, 3 * nnb + g) : C{R(0), R(0)};
         }
       }
 #pragma unroll
@@ -315,9 +317,9 @@ __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __re
       __syncthreads();
       if (own) {
         const unsigned ob = (unsigned)i * plb + colb;
-        FlatAcc<R, NT> a{sm, ws, w, nz, pa, ca, na, k2x_p, k2x_c, k2p_c, k2p_n};
+        FlatAcc<R, NT, WS> a{sm, w, nz, pa, ca, na, k2x_p, k2x_c, k2p_c, k2p_n};
         C rx, ry, rz, rp;
-        rows4<R>(i, j, k, nx, ny, nz, axis_at(op, 0, i), axis_at(op, 1, j), axis_at(op, 2, k), a,
+        rows4<R>(i, j, k, nx, ny, nz, axis_at_g(op, 0, i), axis_at_g(op, 1, j), axis_at_g(op, 2, k), a,
                  rx, ry, rz, rp);
         sto(sout, ob, rx);
         sto(sout, nnb + ob, ry);

@@ -62,6 +62,23 @@ __device__ __forceinline__ void sto(T* __restrict__ base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
+// Loads through pointers that address global memory.  A pointer read from
+// a table in device memory (a batch's operator records) is generic to the
+// compiler, which then issues flat loads; those also count against the LDS
+// counter, so every LDS wait and barrier would wait for them as well.
+template <typename R>
+__device__ __forceinline__ cplx<R> ldg(const cplx<R>* base, unsigned boff) {
+  typedef R V2 __attribute__((ext_vector_type(2)));
+  typedef const __attribute__((address_space(1))) V2 GV2;
+  const V2 v = *(GV2*)(reinterpret_cast<const char*>(base) + boff);
+  return cplx<R>{v.x, v.y};
+}
+template <typename R>
+__device__ __forceinline__ R ldg_r(const R* p) {
+  typedef const __attribute__((address_space(1))) R GR;
+  return *(GR*)p;
+}
+
 // 1-D table values of one axis at one coordinate
 template <typename R>
 struct Ax1 {
@@ -71,6 +88,12 @@ template <typename R>
 __device__ __forceinline__ Ax1<R> axis_at(const OpArgs<R>& op, int a, int q) {
   return {op.nP[a][q], op.nM[a][q], op.eP[a][q], op.eM[a][q],
           op.dN[a][q], op.dE[a][q], op.rdN[a][q], op.rdE[a][q]};
+}
+
+template <typename R>
+__device__ __forceinline__ Ax1<R> axis_at_g(const OpArgs<R>& op, int a, int q) {
+  return {ldg_r(op.nP[a] + q), ldg_r(op.nM[a] + q), ldg_r(op.eP[a] + q), ldg_r(op.eM[a] + q),
+          ldg_r(op.dN[a] + q), ldg_r(op.dE[a] + q), ldg_r(op.rdN[a] + q), ldg_r(op.rdE[a] + q)};
 }
 
 // The four rows of K u at node (i,j,k).  `a` gives field values
