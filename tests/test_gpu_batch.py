@@ -123,6 +123,34 @@ def test_both_batched_stencils_match_the_oracle(gpu_mod, oracle, monkeypatch):
             q += 1
 
 
+@pytest.mark.parametrize("nzc,seed", [(50, 11), (100, 12), (180, 13), (240, 14)])
+def test_batched_stencil_window_strides_match_the_oracle(gpu_mod, oracle, nzc, seed):
+    """The batched x-march is compiled for four LDS window strides, picked by
+    the batch's longest z-row (up to 40, 72, 136 and 228 nodes). Meshes of
+    51, 101 and 181 z-nodes take the last three; 241 z-nodes is past the
+    largest window and takes the one-node-per-thread stencil. Each batch
+    solves to the oracle's fields."""
+    c = meshgen.random_block_case((5, 4, nzc), seed=seed, freq=30.0)
+    nx, ny, nz = c.shape
+    k0 = c.sources[0][3]
+    srcs = [(1, nx // 2, ny // 2, k0), (3, nx // 2, ny // 2 + 1, k0 + 2)]
+    single = gpu_mod.Solver()
+    single.set_mesh(c.xn, c.yn, c.zn)
+    rhs = [single.build_rhs(5, [src])[0] for src in srcs]
+    single.close()
+    with gpu_mod.BatchSolver() as B:
+        op = B.add_case_operator(c)
+        for b in rhs:
+            B.add_system(op, b)
+        reps, _ = B.solve(1e-9, 5000)
+        assert all(r["status"] == 0 and r["true_rel_res"] < 1.05e-9 for r in reps)
+        sols = [flat4(B.solution(q)) for q in range(len(srcs))]
+    for src, xs in zip(srcs, sols):
+        S = oracle.System(meshgen.Case(c.xn, c.yn, c.zn, c.model, c.cells, c.freq, 5, [src]))
+        xo = flat4(S.solve(tol=1e-9, max_it=5000)[0])
+        assert np.linalg.norm(xs - xo) / np.linalg.norm(xo) < 1e-7
+
+
 def test_shared_operators_frequency_sweep_and_warm_restart(gpu_mod):
     """config-1 family: 3 frequencies x 3 dipoles (+X, +Y, +Z at the surface
     centre) share one operator per frequency; then a warm restart from the
