@@ -90,6 +90,19 @@ __device__ __forceinline__ Ax1<R> axis_at(const OpArgs<R>& op, int a, int q) {
           op.dN[a][q], op.dE[a][q], op.rdN[a][q], op.rdE[a][q]};
 }
 
+// Block-uniform reads of the axis tables, which no kernel writes, through
+// the constant address space: the compiler then uses scalar loads into
+// SGPRs, which take no vector-memory slot and no wait on the vector loads.
+template <typename R>
+__device__ __forceinline__ R ldk_r(const R* p) {
+  typedef const __attribute__((address_space(4))) R KR;
+  return *(KR*)p;
+}
+template <typename R>
+__device__ __forceinline__ Ax1<R> axis_at_k(const OpArgs<R>& op, int a, int q) {
+  return {ldk_r(op.nP[a] + q), ldk_r(op.nM[a] + q), ldk_r(op.eP[a] + q), ldk_r(op.eM[a] + q),
+          ldk_r(op.dN[a] + q), ldk_r(op.dE[a] + q), ldk_r(op.rdN[a] + q), ldk_r(op.rdE[a] + q)};
+}
 template <typename R>
 __device__ __forceinline__ Ax1<R> axis_at_g(const OpArgs<R>& op, int a, int q) {
   return {ldg_r(op.nP[a] + q), ldg_r(op.nM[a] + q), ldg_r(op.eP[a] + q), ldg_r(op.eM[a] + q),
@@ -352,7 +365,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
 
     if (own) {
       const unsigned ob = (unsigned)i * plb + colb;
-      const Ax1<R> X = axis_at(op, 0, i);
+      const Ax1<R> X = axis_at_k(op, 0, i);  // i is uniform over the block
       const Ax1<R> Y = {tyj[0][tj], tyj[1][tj], tyj[2][tj], tyj[3][tj],
                         tyj[4][tj], tyj[5][tj], tyj[6][tj], tyj[7][tj]};
       const Ax1<R> Z = {tzk[0][tk], tzk[1][tk], tzk[2][tk], tzk[3][tk],
