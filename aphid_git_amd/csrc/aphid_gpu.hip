@@ -203,6 +203,10 @@ struct aphid_ctx {
   int stall_window = 0;
   double stall_factor = 0.9;
   bool stalled = false;
+  // wall-clock marks of the running solve: (iterations done, seconds since
+  // the solve started) after every host read of the device state
+  std::chrono::steady_clock::time_point t_solve;
+  std::vector<std::pair<int, double>> marks;
 };
 
 namespace {
@@ -348,6 +352,17 @@ struct StreamSet {
 };
 std::map<TuneKey, StreamSet> g_streams;
 
+// A stencil launch shape the kernels support on this context's mesh: one of
+// kTiles, chunk >= 0, and at most kMaxPartBlocks blocks (the capacity of
+// the eps partial-sum buffer).
+bool tile_ok(const aphid_ctx* c, int tk, int tj, int chunk) {
+  if (chunk < 0) return false;
+  bool ok = false;
+  for (const Tile& t : kTiles) ok |= (t.tk == tk && t.tj == tj);
+  return ok && (!c->nn || apply_blocks(Tile{tk, tj, chunk}, c->nx, c->ny, c->nz) <=
+                              (unsigned)kMaxPartBlocks);
+}
+
 // Autotuner: time every (tile, chunk) candidate for this mesh and keep the
 // fastest.  Results are cached per mesh shape and precision in the file
 // named by APHID_TUNE_CACHE (if set), and in memory for the process.
@@ -374,7 +389,9 @@ int autotune(aphid_ctx* c, int prec) {
         if (sscanf(line, "%d %d %d %d %d %d %d", &a, &b, &cc, &p, &tk, &tj, &ch) == 7) {
           char k2[96];
           snprintf(k2, sizeof k2, "%d %d %d %d", a, b, cc, p);
-          if (!strcmp(k2, key)) {
+          // the checks of aphid_set_tile: a line that fails them (stale or
+          // edited) is ignored, and the tile is timed instead
+          if (!strcmp(k2, key) && tile_ok(c, tk, tj, ch)) {
             c->tile[prec] = {tk, tj, ch};
             c->tuned[prec] = true;
           }
@@ -761,6 +778,12 @@ int progress_every() {
   return n;
 }
 
+void mark_time(aphid_ctx* c) {
+  c->marks.push_back(
+      {c->hst->iter,
+       std::chrono::duration<double>(std::chrono::steady_clock::now() - c->t_solve).count()});
+}
+
 void report_progress(const aphid_ctx* c, int& next) {
   const int n = progress_every();
   if (n <= 0) return;
@@ -805,6 +828,7 @@ int run_loop(aphid_ctx* c) {
       }
       CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
       CK(hipStreamSynchronize(c->s));
+      mark_time(c);
       if (c->hst->done) return 0;
       report_progress(c, next_report);
       if (stall.step(c)) {
@@ -844,6 +868,7 @@ int run_loop(aphid_ctx* c) {
     CK(hipGraphLaunch(c->gexec[prec], c->s));
     CK(hipMemcpyAsync(c->hst, c->st, sizeof(QmrState), hipMemcpyDeviceToHost, c->s));
     CK(hipStreamSynchronize(c->s));
+    mark_time(c);
     if (c->hst->done) break;
     report_progress(c, next_report);
     if (stall.step(c)) {
@@ -871,6 +896,8 @@ int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs)
   init.max_it = max_it;
   init.hist_cap = c->hist_cap;
   init.done = 0;
+  c->marks.clear();
+  c->t_solve = std::chrono::steady_clock::now();
   CK(hipMemcpyAsync(c->st, &init, sizeof init, hipMemcpyHostToDevice, c->s));
   CK(hipEventRecord(c->ev0, c->s));
   hipLaunchKernelGGL(k_norm_b<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.b, n4, c->part);
@@ -904,6 +931,19 @@ int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs)
   CK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *secs = 1e-3 * ms;
   return 0;
+}
+
+// Wall-clock seconds of iteration q+1 for q < n: the time of the latest
+// host read of the device state at or before that iteration (0 before the
+// first), like the reference's t2 - t1 column, which it refreshes every 10
+// iterations (src/aphid_qmr.f90:376-381).
+void fill_hist_time(const aphid_ctx* c, double* t, int n) {
+  size_t m = 0;
+  double last = 0.0;
+  for (int q = 0; q < n; ++q) {
+    while (m < c->marks.size() && c->marks[m].first <= q + 1) last = c->marks[m++].second;
+    t[q] = last;
+  }
 }
 
 // ||b - K x|| / ||b|| in complex128 (x, b: c->V[0], c->V[1]; uses V[8])
@@ -1051,10 +1091,19 @@ int aphid_ctx_destroy(aphid_ctx* c) {
   return 0;
 }
 
+// The stencils address a vector of four component blocks through 32-bit
+// byte offsets from its base (component * nn + node) * sizeof(complex128),
+// so 4 * nn * 16 bytes must stay below 2^32: nn < APHID_MAX_NODES.
+bool nodes_fit(int nx, int ny, int nz) {
+  return (long long)nx * ny * nz < (long long)APHID_MAX_NODES;
+}
+
 int aphid_set_mesh(aphid_ctx* c, int nx, int ny, int nz, const double* xn, const double* yn,
                    const double* zn) {
   if (!c || !xn || !yn || !zn) return fail(APHID_E_ARG, "NULL argument");
   if (nx < 3 || ny < 3 || nz < 3) return fail(APHID_E_ARG, "need at least 3 nodes per axis");
+  if (!nodes_fit(nx, ny, nz))
+    return fail(APHID_E_ARG, "mesh too large: 4*nx*ny*nz*16 bytes must stay below 2^32");
   for (int a = 0; a < 3; ++a) {
     const double* v = a == 0 ? xn : (a == 1 ? yn : zn);
     const int n = a == 0 ? nx : (a == 1 ? ny : nz);
@@ -1475,6 +1524,7 @@ int aphid_solve_resident(aphid_ctx* c, double tol, int max_it, unsigned flags,
     rep->hist_len = std::min(std::max(executed, 0), cap);
     if (cap && rep->hist_len > 0)
       CK(hipMemcpy(rep->hist, c->hist, sizeof(double) * rep->hist_len, hipMemcpyDeviceToHost));
+    if (rep->hist_time) fill_hist_time(c, rep->hist_time, rep->hist_len);
     rep->seconds_total =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -1727,11 +1777,7 @@ int aphid_get_tile(aphid_ctx* c, int precision, int* tk, int* tj, int* chunk) {
 
 int aphid_set_tile(aphid_ctx* c, int precision, int tk, int tj, int chunk) {
   if (!c || precision < 0 || precision > 1 || chunk < 0) return fail(APHID_E_ARG, "bad argument");
-  bool ok = false;
-  for (const Tile& t : kTiles) ok |= (t.tk == tk && t.tj == tj);
-  if (!ok) return fail(APHID_E_ARG, "unsupported tile");
-  if (c->nn && apply_blocks(Tile{tk, tj, chunk}, c->nx, c->ny, c->nz) > (unsigned)kMaxPartBlocks)
-    return fail(APHID_E_ARG, "too many blocks for this tile");
+  if (!tile_ok(c, tk, tj, chunk)) return fail(APHID_E_ARG, "unsupported tile, or too many blocks");
   c->tile[precision] = {tk, tj, chunk};
   c->tuned[precision] = true;
   if (c->gexec[precision]) (void)hipGraphExecDestroy(c->gexec[precision]), c->gexec[precision] = nullptr;
@@ -2269,6 +2315,8 @@ int aphid_batch_add_operator(aphid_batch* B, int nx, int ny, int nz, const doubl
                              const float* a, const float* b, const float* c, const float* d) {
   if (!B) return fail(APHID_E_ARG, "NULL batch");
   if (!B->sys.empty()) return fail(APHID_E_STATE, "add every operator before the first system");
+  if (nx < 3 || ny < 3 || nz < 3 || !nodes_fit(nx, ny, nz))
+    return fail(APHID_E_ARG, "need 3 to APHID_MAX_NODES-1 nodes (4*nn*16 bytes below 2^32)");
   aphid_ctx* ctx = nullptr;
   int rc = aphid_ctx_create(&ctx, B->dev);
   if (rc) return rc;

@@ -35,7 +35,7 @@
       integer(c_int), allocatable :: sd(:), si(:), sj(:), sk(:), skipped(:)
       complex(c_double_complex), allocatable :: bx(:,:,:), by(:,:,:)
       complex(c_double_complex), allocatable :: bz(:,:,:), bp(:,:,:)
-      real(c_double), allocatable, target :: hist(:)
+      real(c_double), allocatable, target :: hist(:), htime(:)
       complex(c_double_complex), allocatable :: sig(:,:,:)
       complex(c_double_complex), allocatable :: fbx(:,:,:), fby(:,:,:)
       complex(c_double_complex), allocatable :: fbz(:,:,:), fex(:,:,:)
@@ -173,19 +173,26 @@
       end do
 
 !     solve in place (the reference's rhs_* convention)
-      allocate(hist(max_it + 1))
+      allocate(hist(max_it + 1), htime(max_it + 1))
       rep%hist = c_loc(hist(1))
+      rep%hist_time = c_loc(htime(1))
       rep%hist_cap = max_it + 1
       rc = aphid_qmr_solve(ctx, bx, by, bz, bp, c_null_ptr, c_null_ptr, &
                            c_null_ptr, c_null_ptr, target,              &
                            int(max_it, c_int), 0_c_int, rep)
       call check(rc, 'aphid_qmr_solve')
 
+!     qmr.iter: iteration, residual norm, seconds since the solve started
+!     (src/aphid_qmr.f90:381); the progress line every 10 iterations
+!     (:376-379), printed after the solve since the iterations run on the
+!     device
       open(unit=14, file=trim(wdir)//'/qmr.iter', status='replace')
       do i = 1, rep%hist_len
-        write(14, '(i8,1x,es24.16)') i, hist(i)
+        write(14, '(i8,1x,es24.16,1x,es24.16)') i, hist(i), htime(i)
+        if (mod(i, 10) == 0) write(6, 98) i, max_it, hist(i), htime(i)
       end do
       close(14)
+   98 format('it: ',i6,'/',i6,'    r: ',e13.6,'    t:',e13.6)
 
       select case (rep%status)
         case (0); write(6, '(a,i8,a)') ' Solution converged in ',      &

@@ -22,6 +22,7 @@
         type(c_ptr)    :: hist = c_null_ptr
         integer(c_int) :: hist_cap = 0, hist_len = 0
         integer(c_int) :: outer_iters = 0, reserved = 0
+        type(c_ptr)    :: hist_time = c_null_ptr
       end type aphid_qmr_report
 
       type, bind(C) :: aphid_sys_report

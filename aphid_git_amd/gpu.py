@@ -25,6 +25,10 @@ _lib = None
 F_X0 = 1
 F_NO_TRUE_RES = 2
 F_MIXED = 4
+# meshes need nx*ny*nz < MAX_NODES (APHID_MAX_NODES: 32-bit stencil offsets)
+MAX_NODES = 67108864
+# the stencil tiles (tk, tj) the kernels are built for (kTiles)
+TILES = [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4), (32, 16), (64, 8), (16, 32)]
 WS_PREV = 1
 WS_LSQ = 2
 STATUS = {0: "converged", 1: "max_it", 2: "breakdown"}
@@ -39,7 +43,8 @@ class Report(C.Structure):
                 ("rhsnorm", C.c_double), ("true_rel_res", C.c_double),
                 ("seconds_qmr", C.c_double), ("seconds_total", C.c_double),
                 ("hist", C.POINTER(C.c_double)), ("hist_cap", C.c_int),
-                ("hist_len", C.c_int), ("outer_iters", C.c_int), ("reserved", C.c_int)]
+                ("hist_len", C.c_int), ("outer_iters", C.c_int), ("reserved", C.c_int),
+                ("hist_time", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -92,6 +97,7 @@ class SolveResult:
     seconds_total: float
     outer_iters: int
     hist: np.ndarray
+    hist_time: Optional[np.ndarray] = None
 
     @property
     def converged(self) -> bool:
@@ -219,15 +225,17 @@ class Solver:
             flags |= F_X0
         cap = max_it + 1 if hist_cap is None else hist_cap
         hist = np.zeros(max(cap, 1))
+        htime = np.zeros(max(cap, 1))
         rep = Report()
         rep.hist = hist.ctypes.data_as(C.POINTER(C.c_double))
+        rep.hist_time = htime.ctypes.data_as(C.POINTER(C.c_double))
         rep.hist_cap = cap
         _check(lib().aphid_qmr_solve(self._ctx, *[_p(a) for a in x], *[_p(a) for a in x0a],
                                      C.c_double(tol), C.c_int(max_it), C.c_uint(flags),
                                      C.byref(rep)), "aphid_qmr_solve")
         return SolveResult(x, rep.iters, rep.status, rep.rnorm, rep.rhsnorm, rep.true_rel_res,
                            rep.seconds_qmr, rep.seconds_total, rep.outer_iters,
-                           hist[:rep.hist_len].copy())
+                           hist[:rep.hist_len].copy(), htime[:rep.hist_len].copy())
 
     def load_rhs(self, b, x0=None):
         shp = comp_shapes(*self.shape)

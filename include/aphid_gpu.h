@@ -41,6 +41,13 @@ extern "C" {
 #define APHID_E_STATE (-3)
 #define APHID_E_NOMEM (-4)
 
+/* Largest mesh, in nodes nx*ny*nz (exclusive): every stencil addresses a
+ * vector of four complex128 component blocks through 32-bit byte offsets,
+ * so 4*nn*16 bytes must stay below 2^32 (e.g. 406^3 nodes fit, 407^3 do
+ * not).  aphid_set_mesh and aphid_batch_add_operator return APHID_E_ARG,
+ * before any allocation, for a larger mesh. */
+#define APHID_MAX_NODES 67108864LL
+
 /* aphid_qmr_solve flags */
 #define APHID_F_X0 1u          /* x0 arrays hold an initial guess (else x0 = 0) */
 #define APHID_F_NO_TRUE_RES 2u /* skip the final ||b - A x|| check */
@@ -66,6 +73,11 @@ typedef struct {
   int hist_len;         /* out: entries written */
   int outer_iters;      /* out: refinement sweeps (APHID_F_MIXED), else 1 */
   int reserved;
+  double *hist_time;    /* optional, caller-owned, hist_cap entries: wall
+                           seconds since the solve started, as of the latest
+                           host read of the device state at or before each
+                           iteration (the t2-t1 column of the reference's
+                           qmr.iter, src/aphid_qmr.f90:376-381) */
 } aphid_qmr_report;
 
 /* Context life cycle.  device < 0 selects the current HIP device. */
@@ -76,7 +88,7 @@ const char *aphid_last_error(void);
 /* Mesh from node coordinates: metrics dx(0:n), dxc(1:n) (read_mesh,
  * src/aphid.f90:341-361) and the operator's 1-D coefficient tables
  * (gen_inprod_{x,y,z,p}_coefs, src/aphid.f90:750-797,878-925,1007-1054,
- * 1135-1185).  Requires nx, ny, nz >= 3. */
+ * 1135-1185).  Requires nx, ny, nz >= 3 and nx*ny*nz < APHID_MAX_NODES. */
 int aphid_set_mesh(aphid_ctx *ctx, int nx, int ny, int nz, const double *xn,
                    const double *yn, const double *zn);
 

@@ -69,3 +69,13 @@ def test_driver_solution_matches_oracle(tmp_path, oracle, model):
     it = np.loadtxt(tmp_path / "qmr.iter", ndmin=2)
     assert abs(len(it) - len(hist)) <= max(3, 0.1 * len(hist))
     np.testing.assert_allclose(it[:8, 1], hist[:8], rtol=1e-9)
+    # the reference's three columns (src/aphid_qmr.f90:381): iteration,
+    # residual norm, seconds since the solve started (non-decreasing)
+    assert it.shape[1] == 3
+    np.testing.assert_array_equal(it[:, 0], np.arange(1, len(it) + 1))
+    assert np.all(np.diff(it[:, 2]) >= 0) and np.all(it[:, 2] >= 0)
+    # its progress line every 10 iterations (src/aphid_qmr.f90:376-379)
+    prog = [ln for ln in r.stdout.splitlines() if ln.startswith("it: ")]
+    assert len(prog) == len(it) // 10
+    if prog:
+        assert prog[0].split()[1:3] == ["10/", str(case.max_it)]
