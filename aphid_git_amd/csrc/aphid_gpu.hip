@@ -151,6 +151,40 @@ AxisTables make_axis(const double* c, int n) {
 int aphid_fail(int code, const std::string& msg) { return fail(code, msg); }
 
 // ---------------------------------------------------------------------------
+// Scratch of aphid_apply_noalloc: one file-scope device array per GPU (the
+// device copies of the input and the output vector, 4*nn complex128 each,
+// and the nn-value staging of the layout permutation: 9*nn*16 bytes), held
+// from the code object's load until the process ends.  Host code never
+// passes `scratch` itself to a kernel (in host code the name is only a
+// host-side stand-in): aphid_ctx_create looks up its device address once
+// per device with hipGetSymbolAddress, and every launch and copy uses that
+// address.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device -> address of `scratch` there
+
+// the device address of `scratch` on the current device `dev` (once per
+// device; later calls return the stored address)
+int scratch_address(int dev, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return 0;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(APHID_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+  *out = p;
+  return 0;
+}
+}  // namespace
+
 struct aphid_ctx {
   int dev = 0;
   hipStream_t s = nullptr;
@@ -207,6 +241,7 @@ struct aphid_ctx {
   // the solve started) after every host read of the device state
   std::chrono::steady_clock::time_point t_solve;
   std::vector<std::pair<int, double>> marks;
+  void* scratch = nullptr;  // device address of `scratch` on this context's device
 };
 
 namespace {
@@ -250,29 +285,32 @@ int comp_shape(const aphid_ctx* c, int comp, int& n1, int& n2, int& n3) {
   return 0;
 }
 
-// host Fortran-order component arrays -> device vector (4 blocks of nn)
-int upload4(aphid_ctx* c, const double* const h[4], c128* dst) {
+// host Fortran-order component arrays -> device vector (4 blocks of nn),
+// through a device staging buffer of nn complex values (default c->stage)
+int upload4(aphid_ctx* c, const double* const h[4], c128* dst, c128* stage = nullptr) {
+  if (!stage) stage = c->stage;
   for (int comp = 0; comp < 4; ++comp) {
     int n1, n2, n3;
     comp_shape(c, comp, n1, n2, n3);
     const size_t bytes = sizeof(c128) * (size_t)n1 * n2 * n3;
-    CK(hipMemcpyAsync(c->stage, h[comp], bytes, hipMemcpyHostToDevice, c->s));
-    hipLaunchKernelGGL(k_f2d<double>, dim3(grid_for(c->nn)), dim3(kBlock), 0, c->s, c->stage, n1,
+    CK(hipMemcpyAsync(stage, h[comp], bytes, hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL(k_f2d<double>, dim3(grid_for(c->nn)), dim3(kBlock), 0, c->s, stage, n1,
                        n2, n3, dst + comp * c->nn, dims(c));
     CK(hipGetLastError());
   }
   return 0;
 }
 
-int download4(aphid_ctx* c, const c128* src, double* const h[4]) {
+int download4(aphid_ctx* c, const c128* src, double* const h[4], c128* stage = nullptr) {
+  if (!stage) stage = c->stage;
   for (int comp = 0; comp < 4; ++comp) {
     int n1, n2, n3;
     comp_shape(c, comp, n1, n2, n3);
     const long long n = (long long)n1 * n2 * n3;
     hipLaunchKernelGGL(k_d2f<double>, dim3(grid_for(n)), dim3(kBlock), 0, c->s,
-                       src + comp * c->nn, n1, n2, n3, c->stage, dims(c));
+                       src + comp * c->nn, n1, n2, n3, stage, dims(c));
     CK(hipGetLastError());
-    CK(hipMemcpyAsync(h[comp], c->stage, sizeof(c128) * (size_t)n, hipMemcpyDeviceToHost, c->s));
+    CK(hipMemcpyAsync(h[comp], stage, sizeof(c128) * (size_t)n, hipMemcpyDeviceToHost, c->s));
   }
   CK(hipStreamSynchronize(c->s));
   return 0;
@@ -1059,8 +1097,12 @@ int aphid_ctx_create(aphid_ctx** out, int device) {
   if (device < 0) CK(hipGetDevice(&device));
   if (device >= ndev) return fail(APHID_E_ARG, "device index out of range");
   CK(hipSetDevice(device));
+  void* sp = nullptr;
+  const int src = scratch_address(device, &sp);
+  if (src) return src;
   aphid_ctx* c = new aphid_ctx();
   c->dev = device;
+  c->scratch = sp;
   CK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
   CK(hipMalloc(&c->st, sizeof(QmrState)));
   CK(hipMalloc(&c->tst, sizeof(QmrState)));
@@ -1292,6 +1334,33 @@ int aphid_apply(aphid_ctx* c, const double* ix, const double* iy, const double* 
   CK(launch_apply<double>(c->tile[0], c->op64, c->V[3], c->V[4], ep, c->s));
   double* hout[4] = {ox, oy, oz, op};
   return download4(c, c->V[4], hout);
+}
+
+int aphid_apply_noalloc(aphid_ctx* c, const double* ix, const double* iy, const double* iz,
+                        const double* ip, double* ox, double* oy, double* oz, double* op) {
+  if (!c || !ix || !iy || !iz || !ip || !ox || !oy || !oz || !op)
+    return fail(APHID_E_ARG, "NULL argument");
+  if (!c->have_k2) return fail(APHID_E_STATE, "set k2 (sigma, cells or k2) first");
+  if (9 * (size_t)c->nn * sizeof(c128) > kScratchBytes)
+    return fail(APHID_E_ARG, "mesh too large for aphid_apply_noalloc: 9*nx*ny*nz*16 bytes must "
+                             "not exceed APHID_NOALLOC_SCRATCH_BYTES");
+  if (!c->scratch) return fail(APHID_E_STATE, "no scratch address for this device");
+  CK(hipSetDevice(c->dev));
+  // every entry of the three regions is written before it is read: the
+  // permutation kernel fills all 4*nn input entries (zero padding
+  // included), the stencil all 4*nn outputs, and each copy its staging
+  c128* din = static_cast<c128*>(c->scratch);
+  c128* dout = din + 4 * c->nn;
+  c128* stage = dout + 4 * c->nn;
+  const double* hin[4] = {ix, iy, iz, ip};
+  int rc = upload4(c, hin, din, stage);
+  if (rc) return rc;
+  // the tuned tile if the context has one, else its default: the apply's
+  // outputs do not depend on the launch shape's eps grouping
+  ApplyEpilogue ep{nullptr, nullptr};
+  CK(launch_apply<double>(c->tile[0], c->op64, din, dout, ep, c->s));
+  double* hout[4] = {ox, oy, oz, op};
+  return download4(c, dout, hout, stage);
 }
 
 int aphid_apply_prec(aphid_ctx* c, int precision, const double* ix, const double* iy,

@@ -27,6 +27,8 @@ F_NO_TRUE_RES = 2
 F_MIXED = 4
 # meshes need nx*ny*nz < MAX_NODES (APHID_MAX_NODES: 32-bit stencil offsets)
 MAX_NODES = 67108864
+# scratch of aphid_apply_noalloc per GPU (APHID_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 # the stencil tiles (tk, tj) the kernels are built for (kTiles)
 TILES = [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4), (32, 16), (64, 8), (16, 32)]
 WS_PREV = 1
@@ -194,6 +196,15 @@ class Solver:
         out = self._zeros4()
         _check(lib().aphid_apply(self._ctx, *[_p(a) for a in u], *[_p(a) for a in out]),
                "aphid_apply")
+        return out
+
+    def apply_noalloc(self, u):
+        """K u like apply(), through the library's per-GPU scratch array
+        (no device allocation; meshes up to NOALLOC_SCRATCH_BYTES)."""
+        u = [_c128(a, s) for a, s in zip(u, comp_shapes(*self.shape))]
+        out = self._zeros4()
+        _check(lib().aphid_apply_noalloc(self._ctx, *[_p(a) for a in u], *[_p(a) for a in out]),
+               "aphid_apply_noalloc")
         return out
 
     def apply_prec(self, u, precision: int):
@@ -502,7 +513,7 @@ class UNet:
 
 EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_set_mesh",
            "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
-           "aphid_get_diag", "aphid_apply", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
+           "aphid_get_diag", "aphid_apply", "aphid_apply_noalloc", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
            "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
            "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy",
            "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",

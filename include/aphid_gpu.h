@@ -124,6 +124,23 @@ int aphid_apply(aphid_ctx *ctx, const double *in_x, const double *in_y,
                 const double *in_z, const double *in_p, double *out_x,
                 double *out_y, double *out_z, double *out_p);
 
+/* aphid_apply without any device allocation.  No counterpart in the
+ * reference: this is for callers that cannot allocate device memory (code
+ * under stream/graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, semantics and results as aphid_apply; the
+ * device copies of in and out and the layout staging live in one
+ * library-wide device array of APHID_NOALLOC_SCRATCH_BYTES per GPU instead
+ * of the context's vectors, so the mesh must satisfy
+ * 9*nx*ny*nz*16 <= APHID_NOALLOC_SCRATCH_BYTES (e.g. up to 77^3 nodes), else
+ * APHID_E_ARG.  The operator (mesh, k^2) must already be set on the
+ * context; the call uses the context's tile as it stands (it never runs the
+ * autotuner).  All calls on one GPU share that array, so two calls that
+ * overlap in time on the same GPU (from different host threads) race. */
+#define APHID_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int aphid_apply_noalloc(aphid_ctx *ctx, const double *in_x, const double *in_y,
+                        const double *in_z, const double *in_p, double *out_x,
+                        double *out_y, double *out_z, double *out_p);
+
 /* aphid_apply with the stencil in a chosen precision: 0 = complex128
  * (identical to aphid_apply), 1 = complex64 operator data and arithmetic
  * (the mixed-precision inner operator; inputs and outputs stay
