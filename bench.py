@@ -60,55 +60,91 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, gpu_iters: float, sample_its: int = 30):
-    """Reference QMR (4 OpenMP threads, qmr_mthread) on this host, or the
-    oracle port when the reference harness is not built."""
+def host_cpu():
+    """lscpu model name and the host's logical CPU count."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        model = m.group(1).strip() if m else ""
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model_name": model, "nproc": os.cpu_count()}
+
+
+def _ref_sample(harness, case, cmd: str, threads: int, sample_its: int):
+    """Run the reference harness's QMR (cmd qmr1 = serial qmr, qmr4 =
+    qmr_mthread) for sample_its iterations on the case; seconds per
+    iteration from its qmr.iter clock (column 3, refreshed every 10
+    iterations, src/aphid_qmr.f90:376-381) between iterations 10 and
+    sample_its."""
+    from aphid_git_amd import meshgen
+
+    with tempfile.TemporaryDirectory() as d:
+        c2 = meshgen.Case(case.xn, case.yn, case.zn, case.model, case.cells, case.freq,
+                          case.mode, case.sources, case.target, sample_its, case.comment)
+        meshgen.write_case(c2, d)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        t0 = time.time()
+        r = subprocess.run([harness, cmd], cwd=d, env=env, capture_output=True, text=True,
+                           timeout=900)
+        wall = time.time() - t0
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-500:])
+        it = np.loadtxt(os.path.join(d, "qmr.iter"), ndmin=2)
+        t10 = it[it[:, 0] == 10][0, 2]
+        tN = it[it[:, 0] == sample_its][0, 2]
+        m = re.search(r"Time spent in QMR \[s\]:\s*([0-9.Ee+-]+)", r.stdout)
+    return (tN - t10) / (sample_its - 10), wall, (float(m.group(1)) if m else None)
+
+
+def cpu_baseline(case, gpu_iters: float, sample_its: int = 30, serial_its: int = 20):
+    """The reference's own QMR on this host, both legal thread counts
+    (src/aphid_serial.f90:9-16): qmr_mthread with OMP_NUM_THREADS=4 (the
+    reported value) and the serial qmr with OMP_NUM_THREADS=1; or the oracle
+    port when the reference harness is not built."""
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     M = case.n_unknowns
+    cpu = host_cpu()
     if os.path.exists(harness):
-        with tempfile.TemporaryDirectory() as d:
-            from aphid_git_amd import meshgen
-
-            c2 = meshgen.Case(case.xn, case.yn, case.zn, case.model, case.cells, case.freq,
-                              case.mode, case.sources, case.target, sample_its, case.comment)
-            meshgen.write_case(c2, d)
-            env = dict(os.environ, OMP_NUM_THREADS="4", OMP_PROC_BIND="close")
-            t0 = time.time()
-            r = subprocess.run([harness, "qmr4"], cwd=d, env=env, capture_output=True, text=True,
-                               timeout=900)
-            wall = time.time() - t0
-            if r.returncode != 0:
-                raise RuntimeError(r.stderr[-500:])
-            it = np.loadtxt(os.path.join(d, "qmr.iter"), ndmin=2)
-            # column 3 is the QMR clock, refreshed every 10 iterations
-            t10 = it[it[:, 0] == 10][0, 2]
-            tN = it[it[:, 0] == sample_its][0, 2]
-            s_it = (tN - t10) / (sample_its - 10)
-            m = re.search(r"Time spent in QMR \[s\]:\s*([0-9.Ee+-]+)", r.stdout)
+        s4, wall4, q4 = _ref_sample(harness, case, "qmr4", 4, sample_its)
+        s1, wall1, q1 = _ref_sample(harness, case, "qmr1", 1, serial_its)
         return {
-            "value": 1.0 / (gpu_iters * s_it),
+            "value": 1.0 / (gpu_iters * s4),
             "unit": "converged solves/s",
             "cores": 4,
             "kind": "reference",
-            "sample": (f"reference qmr_mthread (OMP_NUM_THREADS=4) on the same 129x129x97 system, "
-                       f"{sample_its} iterations; {s_it:.4f} s/iteration from its qmr.iter clock "
-                       f"(iterations 10-{sample_its}), extrapolated to the GPU's "
-                       f"{gpu_iters:.0f} iterations per converged solve; harness wall {wall:.1f} s"
-                       + (f", reference 'Time spent in QMR' {float(m.group(1)):.2f} s" if m else "")),
-            "s_per_iteration": s_it,
-            "ns_per_unknown_iteration": 1e9 * s_it / M,
+            "sample": (f"reference qmr_mthread (OMP_NUM_THREADS=4, OMP_PROC_BIND=close) on the "
+                       f"same 129x129x97 system, {sample_its} iterations; {s4:.4f} s/iteration "
+                       f"from its qmr.iter clock (iterations 10-{sample_its}), extrapolated to the "
+                       f"GPU's {gpu_iters:.0f} iterations per converged solve; harness wall "
+                       f"{wall4:.1f} s" + (f", its 'Time spent in QMR' {q4:.2f} s" if q4 else "")),
+            "s_per_iteration": s4,
+            "ns_per_unknown_iteration": 1e9 * s4 / M,
+            "serial": {"value": 1.0 / (gpu_iters * s1), "cores": 1, "s_per_iteration": s1,
+                       "ns_per_unknown_iteration": 1e9 * s1 / M,
+                       "sample": f"reference serial qmr (OMP_NUM_THREADS=1), {serial_its} "
+                                 f"iterations, s/iteration over iterations 10-{serial_its}; "
+                                 f"harness wall {wall1:.1f} s"},
+            "host_cpu": cpu,
             "host": os.uname().nodename,
         }
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     S = O.System(case)
-    x, rep, hist = O.qmr(S.op, S.b, S.diag, 0.0, sample_its, nthreads=4)
-    s_it = rep.seconds / sample_its
-    return {"value": 1.0 / (gpu_iters * s_it), "unit": "converged solves/s", "cores": 4,
-            "kind": "port",
-            "sample": f"oracle port, 4 threads, {sample_its} iterations, extrapolated to "
-                      f"{gpu_iters:.0f} iterations", "s_per_iteration": s_it}
+    out = {"unit": "converged solves/s", "cores": 4, "kind": "port", "host_cpu": cpu}
+    for nt, its in ((4, sample_its), (1, serial_its)):
+        x, rep, hist = O.qmr(S.op, S.b, S.diag, 0.0, its, nthreads=nt)
+        s_it = rep.seconds / its
+        d = {"value": 1.0 / (gpu_iters * s_it), "cores": nt, "s_per_iteration": s_it,
+             "sample": f"oracle port, {nt} thread(s), {its} iterations, extrapolated to "
+                       f"{gpu_iters:.0f} iterations"}
+        if nt == 4:
+            out.update(d)
+        else:
+            out["serial"] = d
+    return out
 
 
 def bench_batch():
@@ -401,7 +437,12 @@ def main():
         "data": "synthetic (seeded block model, config2)",
         "config": {"workload": "config2: 128x128x96-cell stretched mesh, 64x64x48 core at 25 m, "
                                "20 random blocks, +X dipole, 1 Hz, Jacobi-QMR to rel-res 1e-8",
-                   "nodes": [nx, ny, nz], "unknowns": M, "tol": TOL},
+                   "nodes": [nx, ny, nz], "unknowns": M, "tol": TOL,
+                   "value_definition": "converged solves per second of wall time over the timed "
+                                       "steps, each a complete solve from x0 = 0 with the "
+                                       "operator and b resident in HBM (aphid_solve_resident); "
+                                       "converged = true ||b - K x|| / ||b|| <= 1e-8. "
+                                       "end_to_end adds setup, upload and download"},
     }
     iters = None
     if "solve" in parts:
@@ -453,13 +494,23 @@ def main():
             # PMC traffic of the same kernel and tile only
             if list(tj.get("tile", [])) == list(tile):
                 traffic = tj.get("stencil_bytes_per_launch")
+        # the roofline entry times the stencil as the iteration launches it
+        # (with the eps = p^T K p epilogue, inside the iteration's own kernel
+        # sequence); the plain apply alone is reported beside it
+        t_st = kt[1]
         out.update({
             "value": converged / elapsed,
             "ms_per_step": 1e3 * elapsed / args.steps,
-            "roofline": {"bound": "hbm", "kernel": "stencil (k_apply, matvec)",
-                         "achieved": B_mv / t_apply / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": B_mv / t_apply / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                         "algorithmic_bytes_per_launch": B_mv, "seconds_per_launch": t_apply,
+            "roofline": {"bound": "hbm", "kernel": "stencil (k_apply with the eps epilogue, "
+                                                   "in the QMR iteration)",
+                         "achieved": B_mv / t_st / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": B_mv / t_st / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                         "traffic_source": "profiles/traffic_config2.json: rocprofv3 --pmc "
+                                           "FETCH_SIZE and WRITE_SIZE of the same kernel and "
+                                           "tile, scaled by a copy kernel's counts",
+                         "algorithmic_bytes_per_launch": B_mv, "seconds_per_launch": t_st,
+                         "plain_apply_seconds": t_apply,
+                         "plain_apply_frac": B_mv / t_apply / 1e9 / HBM_PEAK_GBS,
                          "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]}},
             "roofline_dominant_kernel": {"kernel": dom, **kernels[dom], "peak": HBM_PEAK_GBS},
             "kernels": kernels,
@@ -526,7 +577,7 @@ def main():
         log(f"[bench] warm: {out['warm_start']}")
     cpu = None
     if "cpu" in parts and iters is not None:
-        log("[bench] CPU baseline (reference, 4 threads) ...")
+        log("[bench] CPU baseline (reference, 4 threads and serial) ...")
         try:
             cpu = cpu_baseline(case, iters, args.cpu_its)
         except Exception as e:  # reported, never fatal
