@@ -1987,6 +1987,10 @@ struct aphid_batch {
     long long off, n4;
     bool x0;
     c128* staged;  // b (and x0) on the device until the flat arrays exist
+    // second-wave system of a sweep: its initial guess is fitted from the
+    // solutions of these systems (aphid_batch_set_sweep_guess)
+    std::vector<int> from;
+    double r0 = 1.0;  // ||b - K x0|| / ||b|| of its initial guess (NaN: not formed)
   };
   std::vector<Sys> sys;
   long long n4_tot = 0, op4_tot = 0;
@@ -2408,7 +2412,7 @@ int aphid_batch_add_system(aphid_batch* B, int op, const double* bx, const doubl
   const bool have_x0 = x0x || x0y || x0z || x0p;
   if (have_x0 && !(x0x && x0y && x0z && x0p)) return fail(APHID_E_ARG, "x0 arrays incomplete");
   if (B->built) return fail(APHID_E_STATE, "add every system before the first solve");
-  B->sys.push_back({op, 0, 0, have_x0, nullptr});
+  B->sys.push_back({op, 0, 0, have_x0, nullptr, {}, have_x0 ? NAN : 1.0});
   // upload right-hand side and guess into host-side staging until the build
   aphid_ctx* c = B->ops[op];
   const long long n4 = 4 * c->nn;
@@ -2567,6 +2571,105 @@ int batch_solve_single(aphid_batch* B, int q, double tol, int max_it, bool use_x
   return 0;
 }
 
+// Solve the systems with in_wave[q] != 0 (complex128): those of at least
+// batch_seq_min() unknowns one at a time on the single-system path, the
+// rest in the batched launches with every other system masked out.
+int batch_solve_wave(aphid_batch* B, const std::vector<int>& in_wave, double tol, int max_it,
+                     bool use_x0, std::vector<QmrState>& last, std::vector<int>& iters) {
+  const int ns = (int)B->sys.size();
+  const long long seq_min = batch_seq_min();
+  std::vector<int> skip(ns, 1);
+  int nbat = 0, nseq = 0;
+  for (int q = 0; q < ns; ++q)
+    if (in_wave[q]) {
+      if (B->sys[q].n4 >= seq_min) ++nseq;
+      else skip[q] = 0, ++nbat;
+    }
+  if (nseq) {
+    CK(hipStreamSynchronize(B->s));  // b, x0 and the zeroed x are in place
+    for (int q = 0; q < ns; ++q)
+      if (in_wave[q] && B->sys[q].n4 >= seq_min) {
+        const int rc = batch_solve_single(B, q, tol, max_it, use_x0, &last[q]);
+        if (rc) return rc;
+        iters[q] = last[q].iter;
+      }
+  }
+  if (nbat) {
+    if (!B->dskip) CK(hipMalloc(&B->dskip, sizeof(int) * ns));
+    CK(hipMemcpyAsync(B->dskip, skip.data(), sizeof(int) * ns, hipMemcpyHostToDevice, B->s));
+    const int rc = batch_qmr<double>(B, tol, nullptr, max_it, use_x0, nullptr, B->dskip);
+    if (rc) return rc;
+    for (int q = 0; q < ns; ++q)
+      if (!skip[q]) {
+        last[q] = B->hst[q];
+        iters[q] = last[q].iter;
+      }
+  }
+  return 0;
+}
+
+// Initial guess of system q from the current solutions of the systems in
+// B->sys[q].from (same mesh), x0 = sum_a al_a x_a, with the amplitudes from
+// the Galerkin condition x_a^T (b - K x0) = 0 on q's own operator (the
+// bilinear form of QMR), or, when that leaves ||b - K x0|| > ||b||, by
+// least squares on the residual; x0 = 0 if neither fit is usable.  Runs on
+// q's operator context, which borrows q's slices of the flat vectors (x, b
+// and the spare Krylov vectors, which the solve rewrites before reading).
+int batch_fit_guess(aphid_batch* B, int q) {
+  auto& sy = B->sys[q];
+  aphid_ctx* c = B->ops[sy.op];
+  if (!c->vborrowed)
+    for (auto& v : c->V)
+      if (v) (void)hipFree(v), v = nullptr;
+  for (int i = 0; i < 9; ++i) c->V[i] = B->V[i] + sy.off;
+  c->vborrowed = true;
+  int rc = autotune(c, 0);
+  if (rc) return rc;
+  const long long nn = c->nn, n4 = 4 * nn;
+  const ApplyEpilogue ep{nullptr, nullptr};
+  GramArgs g{};
+  g.b = c->V[1];
+  g.n = n4;
+  g.nn = nn;
+  CombineArgs cb{};
+  for (int from : sy.from) {
+    const int a = g.nb++;
+    const c128* u = B->V[0] + B->sys[from].off;
+    CK(launch_apply<double>(c->tile[0], c->op64, u, c->V[2 + a], ep, c->s));  // K u_a
+    cb.u[a] = u;
+    g.u[a] = u;
+    g.ublk[a] = -1;
+    g.w[a] = c->V[2 + a];
+  }
+  cb.nb = g.nb;
+  double rel = 1.0, rn = 0.0, bn = 0.0;
+  bool lsq = false;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    std::vector<c128> al;
+    bool ok = false;
+    rc = ws_fit(c, g, lsq, al, &ok);
+    if (rc) return rc;
+    if (ok) {
+      for (int a = 0; a < g.nb; ++a) cb.al[a] = al[a];
+      hipLaunchKernelGGL(k_combine, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, n4, cb, c->V[0]);
+      CK(hipGetLastError());
+    } else {
+      CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
+    }
+    rc = true_residual(c, &rel, &rn, &bn);
+    if (rc) return rc;
+    if (std::isfinite(rel) && (rel <= 1.0 || lsq)) break;
+    lsq = true;
+  }
+  if (!std::isfinite(rel) || rel > 1.0) {
+    CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
+    CK(hipStreamSynchronize(c->s));
+    rel = 1.0;
+  }
+  sy.r0 = rel;
+  return 0;
+}
+
 // per-system ||b - K x|| and ||b|| (complex128) into B->res
 int batch_true_residual(aphid_batch* B) {
   const BatchDev D = batch_dev(B);
@@ -2612,37 +2715,37 @@ int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
   const bool use_x0 = (flags & APHID_F_X0) || any_x0;
   if (!use_x0) CK(hipMemsetAsync(B->V[0], 0, sizeof(c128) * B->n4_tot, B->s));
   B->solved = true;
-  CK(hipEventRecord(B->ev0, B->s));
+  CK(hipStreamSynchronize(B->s));
+  // wall clock: large systems run on their operator's own stream
+  const auto t_start = std::chrono::steady_clock::now();
   std::vector<int> iters(ns, 0), sweeps(ns, 0);
   std::vector<QmrState> last(ns);
   int rc = 0;
   if (!(flags & APHID_F_MIXED)) {
-    // large systems one at a time on the single-system path, the rest in
-    // the batched launches with the large ones masked out
-    const long long seq_min = batch_seq_min();
-    std::vector<int> skip(ns, 0);
-    int nseq = 0;
-    for (int q = 0; q < ns; ++q)
-      if (B->sys[q].n4 >= seq_min) skip[q] = 1, ++nseq;
-    if (nseq) {
-      CK(hipStreamSynchronize(B->s));  // b, x0 and the zeroed x are in place
-      for (int q = 0; q < ns; ++q)
-        if (skip[q]) {
-          rc = batch_solve_single(B, q, tol, max_it, use_x0, &last[q]);
-          if (rc) return rc;
-          iters[q] = last[q].iter;
-        }
+    // one wave, or a sweep: first every system without a sweep guess, then
+    // the others from initial guesses fitted to the first wave's solutions
+    std::vector<int> w0(ns), w1(ns);
+    bool sweep = false;
+    for (int q = 0; q < ns; ++q) {
+      const bool second = !B->sys[q].from.empty();
+      w0[q] = !second;
+      w1[q] = second;
+      sweep |= second;
+      if (!second) B->sys[q].r0 = use_x0 ? NAN : 1.0;
     }
-    if (nseq < ns) {
-      if (!B->dskip) CK(hipMalloc(&B->dskip, sizeof(int) * ns));
-      CK(hipMemcpyAsync(B->dskip, skip.data(), sizeof(int) * ns, hipMemcpyHostToDevice, B->s));
-      rc = batch_qmr<double>(B, tol, nullptr, max_it, use_x0, nullptr, B->dskip);
-      if (rc) return rc;
+    rc = batch_solve_wave(B, w0, tol, max_it, use_x0, last, iters);
+    if (rc) return rc;
+    if (sweep) {
+      CK(hipStreamSynchronize(B->s));
       for (int q = 0; q < ns; ++q)
-        if (!skip[q]) {
-          last[q] = B->hst[q];
-          iters[q] = last[q].iter;
+        if (w1[q]) {
+          rc = batch_fit_guess(B, q);
+          if (rc) return rc;
         }
+      for (int q = 0; q < ns; ++q)
+        if (w1[q]) CK(hipStreamSynchronize(B->ops[B->sys[q].op]->s));
+      rc = batch_solve_wave(B, w1, tol, max_it, true, last, iters);
+      if (rc) return rc;
     }
   } else {
     rc = batch_build_f32(B);
@@ -2689,10 +2792,9 @@ int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
       CK(hipGetLastError());
     }
   }
-  CK(hipEventRecord(B->ev1, B->s));
-  CK(hipEventSynchronize(B->ev1));
-  float ms = 0.f;
-  CK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+  CK(hipStreamSynchronize(B->s));
+  const double wall =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   rc = batch_true_residual(B);
   if (rc) return rc;
   std::vector<double> res(2 * ns);
@@ -2710,10 +2812,11 @@ int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
       reports[q].rnorm = (flags & APHID_F_MIXED) ? res[2 * q] : last[q].rnorm;
       reports[q].rhsnorm = res[2 * q + 1];
       reports[q].true_rel_res = rel;
+      reports[q].r0_rel = (flags & APHID_F_MIXED) ? NAN : B->sys[q].r0;
     }
     worst = std::max(worst, status);
   }
-  if (seconds) *seconds = 1e-3 * ms;
+  if (seconds) *seconds = wall;
   return worst;
 }
 
@@ -2725,6 +2828,29 @@ int aphid_batch_get_solution(aphid_batch* B, int sys, double* x_x, double* x_y, 
   aphid_ctx* c = B->ops[B->sys[sys].op];
   double* h[4] = {x_x, x_y, x_z, x_p};
   return download4(c, B->V[0] + B->sys[sys].off, h);
+}
+
+int aphid_batch_set_sweep_guess(aphid_batch* B, int sys, int nfrom, const int* from) {
+  if (!B) return fail(APHID_E_ARG, "NULL batch");
+  const int ns = (int)B->sys.size();
+  if (sys < 0 || sys >= ns) return fail(APHID_E_ARG, "no such system");
+  if (nfrom < 0 || nfrom > APHID_SWEEP_MAX_FROM || (nfrom > 0 && !from))
+    return fail(APHID_E_ARG, "nfrom must be 0..APHID_SWEEP_MAX_FROM");
+  const aphid_ctx* cs = B->ops[B->sys[sys].op];
+  for (int a = 0; a < nfrom; ++a) {
+    const int f = from[a];
+    if (f < 0 || f >= ns || f == sys) return fail(APHID_E_ARG, "no such source system");
+    const aphid_ctx* cf = B->ops[B->sys[f].op];
+    if (cf->nx != cs->nx || cf->ny != cs->ny || cf->nz != cs->nz)
+      return fail(APHID_E_ARG, "sweep guesses come from systems on the same mesh");
+    if (!B->sys[f].from.empty())
+      return fail(APHID_E_ARG, "a source system must be in the first wave");
+  }
+  for (int q = 0; q < ns && nfrom > 0; ++q)
+    for (int f : B->sys[q].from)
+      if (f == sys) return fail(APHID_E_ARG, "system is a source of another system's guess");
+  B->sys[sys].from.assign(from, from + nfrom);
+  return 0;
 }
 
 int aphid_batch_set_guess(aphid_batch* B, int sys, int from_sys) {

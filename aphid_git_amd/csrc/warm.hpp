@@ -89,6 +89,21 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g, double* part) {
   block_partial<2 * kWsPairs>(acc, part);
 }
 
+// x = sum_a al[a] u[a] over whole vectors (a < nb)
+struct CombineArgs {
+  const c128* u[kWsMax];
+  c128 al[kWsMax];
+  int nb;
+};
+__global__ void k_combine(long long n, CombineArgs g, c128* __restrict__ x) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    c128 v = {0.0, 0.0};
+    for (int a = 0; a < g.nb; ++a) v += g.al[a] * g.u[a][q];
+    x[q] = v;
+  }
+}
+
 struct WsCoef {
   c128 a[4];  // amplitude of each component of the mapped prediction
   c128 prev;  // amplitude of the previous solution

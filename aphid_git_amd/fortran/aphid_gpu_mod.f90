@@ -27,7 +27,7 @@
 
       type, bind(C) :: aphid_sys_report
         integer(c_int) :: iters, status
-        real(c_double) :: rnorm, rhsnorm, true_rel_res
+        real(c_double) :: rnorm, rhsnorm, true_rel_res, r0_rel
       end type aphid_sys_report
 
       interface

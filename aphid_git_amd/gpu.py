@@ -334,7 +334,7 @@ class Solver:
 
 class SysReport(C.Structure):
     _fields_ = [("iters", C.c_int), ("status", C.c_int), ("rnorm", C.c_double),
-                ("rhsnorm", C.c_double), ("true_rel_res", C.c_double)]
+                ("rhsnorm", C.c_double), ("true_rel_res", C.c_double), ("r0_rel", C.c_double)]
 
 
 class BatchSolver:
@@ -402,7 +402,7 @@ class BatchSolver:
         _check(lib().aphid_batch_solve(self._b, C.c_double(tol), C.c_int(max_it), C.c_uint(flags),
                                        reps, C.byref(secs)), "aphid_batch_solve")
         return [dict(iters=r.iters, status=r.status, rnorm=r.rnorm, rhsnorm=r.rhsnorm,
-                     true_rel_res=r.true_rel_res) for r in reps], secs.value
+                     true_rel_res=r.true_rel_res, r0_rel=r.r0_rel) for r in reps], secs.value
 
     def solution(self, sys: int):
         out = [np.zeros(s, np.complex128, order="F")
@@ -413,6 +413,14 @@ class BatchSolver:
 
     def set_guess(self, sys: int, from_sys: int):
         _check(lib().aphid_batch_set_guess(self._b, sys, from_sys), "aphid_batch_set_guess")
+
+    def set_sweep_guess(self, sys: int, from_sys: Sequence[int]):
+        """Second-wave system: its initial guess is fitted (Galerkin) to
+        the solutions of from_sys (first-wave systems on the same mesh)."""
+        arr = np.ascontiguousarray(list(from_sys), np.int32)
+        _check(lib().aphid_batch_set_sweep_guess(self._b, sys, len(arr),
+                                                 _p(arr) if len(arr) else None),
+               "aphid_batch_set_sweep_guess")
 
 
 class UNet:
@@ -518,7 +526,7 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy",
            "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",
            "aphid_batch_add_system", "aphid_batch_solve", "aphid_batch_get_solution",
-           "aphid_batch_set_guess",
+           "aphid_batch_set_guess", "aphid_batch_set_sweep_guess",
            "aphid_unet_create", "aphid_unet_destroy", "aphid_unet_param_count",
            "aphid_unet_forward_flops", "aphid_unet_get_params", "aphid_unet_set_params",
            "aphid_unet_forward", "aphid_unet_loss_grad", "aphid_unet_activation",

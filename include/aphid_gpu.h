@@ -221,6 +221,9 @@ typedef struct {
   double rnorm;        /* final recurrence residual norm */
   double rhsnorm;      /* ||b|| */
   double true_rel_res; /* ||b - K x|| / ||b|| in complex128 */
+  double r0_rel;       /* ||b - K x0|| / ||b|| of the initial guess: 1 from
+                          x0 = 0, the fit's value for a sweep guess, NaN
+                          for a caller's x0 (not formed) or a mixed solve */
 } aphid_sys_report;
 
 int aphid_batch_create(aphid_batch **batch, int device);
@@ -241,7 +244,9 @@ int aphid_batch_add_system(aphid_batch *batch, int op, const double *bx_c,
                            const double *x0_y, const double *x0_z,
                            const double *x0_p);
 /* Solve every system.  reports (optional) has one entry per system; returns
- * the worst status.  seconds (optional): device time of the solve.  With
+ * the worst status.  seconds (optional): wall time of the solve, from the
+ * right-hand sides in place to the last system done (the final true
+ * residuals excluded).  With
  * APHID_F_X0 the current solutions are the initial guesses (warm restart,
  * e.g. after aphid_batch_set_guess). */
 int aphid_batch_solve(aphid_batch *batch, double tol, int max_it,
@@ -252,6 +257,19 @@ int aphid_batch_get_solution(aphid_batch *batch, int sys, double *x_x,
 /* Use system from_sys's current solution as the initial guess of system
  * sys (same mesh), e.g. the previous frequency of a sweep. */
 int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
+/* Sweep schedule (no counterpart in the reference, whose farm solves every
+ * system from x0 = 0, src/aphid_parallel.f90:201 -> src/aphid_qmr.f90:
+ * 230-233): mark system sys as second-wave.  aphid_batch_solve then solves
+ * every unmarked system first, fits sys's initial guess to the solutions of
+ * the nfrom (<= APHID_SWEEP_MAX_FROM) systems `from` -- same mesh, e.g. the
+ * neighbouring frequencies with the same source -- by the Galerkin
+ * condition x_a^T (b - K x0) = 0 on sys's operator (least squares on the
+ * residual if that is worse than x0 = 0, and x0 = 0 if both are), and
+ * solves the marked systems from those guesses (an x0 given for them is
+ * replaced).  Source systems must be unmarked.  nfrom = 0 unmarks. */
+#define APHID_SWEEP_MAX_FROM 4
+int aphid_batch_set_sweep_guess(aphid_batch *batch, int sys, int nfrom,
+                                const int *from);
 
 /* Post-processing of the resident solution (gen_fields, src/aphid.f90:
  * 2173-2252): B = curl A on the primary cell faces -- Bx (nx,ny-1,nz-1),

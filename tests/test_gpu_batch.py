@@ -274,3 +274,187 @@ def test_large_systems_take_the_single_path_in_place(gpu_mod, monkeypatch):
         assert all(r["status"] == 0 and r["true_rel_res"] < 1e-9 and r["iters"] <= 10
                    for r in reps2), reps2
     single.close()
+
+
+def _sweep_family(nf=9, seed=61):
+    """One 16^3-cell random-block model at nf log-spaced frequencies
+    (1e-1..1e3 Hz) x two dipoles: systems ordered frequency-major."""
+    base = meshgen.random_block_case((16, 16, 16), seed=seed, freq=1.0)
+    nx, ny, nz = base.shape
+    k0 = base.sources[0][3]
+    srcs = [(1, nx // 2, ny // 2, k0), (2, nx // 2 - 2, ny // 2 + 1, k0)]
+    return base, np.logspace(-1, 3, nf), srcs
+
+
+@pytest.mark.parametrize("large", [False, True])
+def test_sweep_guess_from_neighbouring_frequencies(gpu_mod, oracle, monkeypatch, large):
+    """Two-wave sweep: even frequencies from x0 = 0, then each odd one from
+    its neighbours' solutions (same source), amplitudes fitted on its own
+    operator.  Every system converges (true rel-res 1e-10) to the oracle's
+    fields (1e-7); each second-wave guess's reported ||b - K x0|| / ||b||
+    equals the host's Galerkin fit of the same candidates (1e-6) and is
+    below 1; the second wave takes fewer iterations than from zero.  With
+    `large` every system takes the single-system path inside the batch."""
+    if large:
+        monkeypatch.setenv("APHID_BATCH_SEQ_MIN", "1")
+    base, freqs, srcs = _sweep_family()
+    single = gpu_mod.Solver()
+    single.set_mesh(base.xn, base.yn, base.zn)
+    rhs = [single.build_rhs(5, [s])[0] for s in srcs]
+    ns = len(srcs)
+    tol = 1e-10
+
+    def build(B):
+        ops = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells)
+               for f in freqs]
+        for o in ops:
+            for b in rhs:
+                B.add_system(o, b)
+
+    with gpu_mod.BatchSolver() as B:
+        build(B)
+        zero, _ = B.solve(tol, 20000)
+    second = {}
+    with gpu_mod.BatchSolver() as B:
+        build(B)
+        for k in range(1, len(freqs), 2):
+            for s in range(ns):
+                nb = [k - 1] + ([k + 1] if k + 1 < len(freqs) else [])
+                second[k * ns + s] = [kk * ns + s for kk in nb]
+                B.set_sweep_guess(k * ns + s, second[k * ns + s])
+        reps, secs = B.solve(tol, 20000)
+        sols = [flat4(B.solution(q)) for q in range(len(reps))]
+    print(f"[sweep large={large}] iters zero {[r['iters'] for r in zero]}", flush=True)
+    print(f"[sweep large={large}] iters swept {[r['iters'] for r in reps]} r0 "
+          f"{['%.2e' % r['r0_rel'] for r in reps]}", flush=True)
+    for q, r in enumerate(reps):
+        assert r["status"] == 0 and r["true_rel_res"] <= tol * 1.05, (q, r)
+        k, s = divmod(q, ns)
+        case = meshgen.Case(base.xn, base.yn, base.zn, "R", base.cells, float(freqs[k]), 5,
+                            [srcs[s]], tol, 20000)
+        xo = flat4(oracle.System(case).solve(tol=tol, max_it=20000)[0])
+        assert np.linalg.norm(sols[q] - xo) / np.linalg.norm(xo) < 1e-7, q
+        if q not in second:
+            assert r["r0_rel"] == 1.0
+            continue
+        # the host's Galerkin fit of the same candidates on q's operator
+        single.set_cells("R", 2 * np.pi * freqs[k], base.cells)
+        U = [split_like(sols[f], base.shape) for f in second[q]]
+        KU = [flat4(single.apply(u)) for u in U]
+        Uf = [flat4(u) for u in U]
+        bq = flat4(rhs[s])
+        G = np.array([[Uf[a] @ KU[c] for c in range(len(U))] for a in range(len(U))])
+        h = np.array([Uf[a] @ bq for a in range(len(U))])
+        al = np.linalg.solve(G, h)
+        r0 = np.linalg.norm(bq - sum(a * ku for a, ku in zip(al, KU))) / np.linalg.norm(bq)
+        if r0 > 1.0:  # the library then falls back to least squares
+            A = np.stack(KU, 1)
+            al = np.linalg.lstsq(A, bq, rcond=None)[0]
+            r0 = np.linalg.norm(bq - A @ al) / np.linalg.norm(bq)
+        assert r["r0_rel"] < 1.0 and abs(r["r0_rel"] - r0) <= 1e-6 * r0, (q, r["r0_rel"], r0)
+    it_zero = sum(zero[q]["iters"] for q in second)
+    it_swept = sum(reps[q]["iters"] for q in second)
+    print(f"[sweep large={large}] second wave: {it_swept} iterations vs {it_zero} from zero",
+          flush=True)
+    assert it_swept < it_zero
+    single.close()
+
+
+def split_like(flat, nodes):
+    from conftest import split4
+
+    return split4(flat, *nodes)[0]
+
+
+def test_sweep_guess_arguments(gpu_mod):
+    base, freqs, srcs = _sweep_family(nf=3)
+    single = gpu_mod.Solver()
+    single.set_mesh(base.xn, base.yn, base.zn)
+    b = single.build_rhs(5, [srcs[0]])[0]
+    single.close()
+    other = meshgen.random_block_case((12, 12, 12), seed=3, freq=1.0)
+    with gpu_mod.BatchSolver() as B:
+        ops = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells)
+               for f in freqs]
+        o2 = B.add_case_operator(other)
+        for o in ops:
+            B.add_system(o, b)
+        s2 = single_rhs_system(gpu_mod, B, o2, other)
+        with pytest.raises(gpu_mod.AphidError, match="same mesh"):
+            B.set_sweep_guess(1, [s2])
+        with pytest.raises(gpu_mod.AphidError, match="source system"):
+            B.set_sweep_guess(1, [1])
+        B.set_sweep_guess(1, [0, 2])
+        with pytest.raises(gpu_mod.AphidError, match="first wave"):
+            B.set_sweep_guess(2, [1])
+        with pytest.raises(gpu_mod.AphidError, match="source of another"):
+            B.set_sweep_guess(0, [2])
+        with pytest.raises(gpu_mod.AphidError, match="nfrom"):
+            B.set_sweep_guess(1, [0, 2, 0, 2, 0])
+
+
+def single_rhs_system(gpu_mod, B, op, case):
+    s = gpu_mod.Solver()
+    b = s.setup_case(case)
+    s.close()
+    return B.add_system(op, b)
+
+
+def test_config3a_combined_sweep_fields_vs_oracle(gpu_mod, oracle):
+    """The config-1 family at 64 frequencies (1e-2..1e5 Hz) x 3 dipoles,
+    batched, complex128, two-wave sweep (odd frequencies from their
+    neighbours' solutions): every system converges to true rel-res 1e-8.
+    On a sample of second-wave systems across the band the fields are within
+    1e-6 of the oracle's converged complex128 solution (rel-res 1e-13).  The
+    one-wave batched solve equals the oracle's own rel-res-1e-8 solve (the
+    same Krylov process from x0 = 0) to rounding; the distance of that
+    solve to the converged solution, printed beside, is what rel-res 1e-8
+    leaves on this family (up to 2.5e-6 near 360 Hz)."""
+    base = meshgen.config1()
+    freqs = np.logspace(-2, 5, 64)
+    rng = np.random.default_rng(3)
+    nx, ny, nz = base.shape
+    pos = [(int(rng.integers(4, nx - 4)), int(rng.integers(4, ny - 4))) for _ in range(3)]
+    srcs = [(1, pos[0][0], pos[0][1], 9), (2, pos[1][0], pos[1][1], 9), (3, pos[2][0], pos[2][1], 9)]
+    single = gpu_mod.Solver()
+    single.set_mesh(base.xn, base.yn, base.zn)
+    bs = [single.build_rhs(5, [q])[0] for q in srcs]
+    single.close()
+    sample = [3 * k + (k // 2) % 3 for k in range(1, 64, 8)]
+    sols, iters = {}, {}
+    for sweep in (False, True):
+        with gpu_mod.BatchSolver() as B:
+            ops = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells)
+                   for f in freqs]
+            for o in ops:
+                for b in bs:
+                    B.add_system(o, b)
+            if sweep:
+                for k in range(1, 64, 2):
+                    for q in range(3):
+                        nb = [k - 1] + ([k + 1] if k + 1 < 64 else [])
+                        B.set_sweep_guess(3 * k + q, [3 * kk + q for kk in nb])
+            reps, _ = B.solve(1e-8, 20000)
+            assert all(r["status"] == 0 and r["true_rel_res"] <= 1e-8 for r in reps)
+            sols[sweep] = {q: flat4(B.solution(q)) for q in sample}
+            iters[sweep] = [r["iters"] for r in reps]
+    odd = [3 * k + q for k in range(1, 64, 2) for q in range(3)]
+    print(f"[3a sweep] second wave: {np.mean([iters[True][q] for q in odd]):.1f} iterations "
+          f"(from zero {np.mean([iters[False][q] for q in odd]):.1f})", flush=True)
+    worst = 0.0
+    for q in sample:
+        k, s = divmod(q, 3)
+        case = meshgen.Case(base.xn, base.yn, base.zn, "R", base.cells, float(freqs[k]), 5,
+                            [srcs[s]], 1e-8, 20000)
+        S = oracle.System(case)
+        x8 = flat4(S.solve(tol=1e-8, max_it=20000, nthreads=4)[0])
+        xc = flat4(S.solve(tol=1e-13, max_it=40000, nthreads=4)[0])
+        e_sw = np.linalg.norm(sols[True][q] - xc) / np.linalg.norm(xc)
+        e_one = np.linalg.norm(sols[False][q] - x8) / np.linalg.norm(x8)
+        e_8 = np.linalg.norm(x8 - xc) / np.linalg.norm(xc)
+        print(f"[3a sweep] system {q} ({freqs[k]:.3g} Hz, dipole {s}): sweep vs converged "
+              f"oracle {e_sw:.2e}; one-wave vs oracle 1e-8 solve {e_one:.2e}; oracle 1e-8 vs "
+              f"converged {e_8:.2e}", flush=True)
+        assert e_one < 1e-10
+        worst = max(worst, e_sw)
+    assert worst < 1e-6, worst
