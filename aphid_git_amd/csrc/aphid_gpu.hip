@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <random>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1675,6 +1676,85 @@ int ws_fit(aphid_ctx* c, const GramArgs& g, bool lsq, std::vector<c128>& al, boo
   return 0;
 }
 
+// The network's output `out` (device, [cells][8]) mapped onto the unknowns
+// (u, in V[7]) and its four components u_c = P_c u fitted by least squares
+// on the residual, weighted (jacobi: W = 1/diag, the M^-1 of QMR) or not
+// (W = 1):  al = argmin || W (b - sum_c al_c K P_c u) ||.
+// Leaves w_c = W K P_c u in V[2..5] and W b in V[6]; uses V[8].
+int net_wfit(aphid_ctx* c, const float* out, bool jacobi, std::vector<c128>& al, bool* ok) {
+  const long long nn = c->nn, n4 = 4 * nn;
+  const ApplyEpilogue ep{nullptr, nullptr};
+  hipLaunchKernelGGL(k_cells_to_unknowns, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, c->nx, c->ny,
+                     c->nz, out, c->V[7]);
+  CK(hipGetLastError());
+  GramArgs g{};
+  for (int q = 0; q < 4; ++q) {
+    CK(hipMemsetAsync(c->V[8], 0, sizeof(c128) * n4, c->s));
+    CK(hipMemcpyAsync(c->V[8] + q * nn, c->V[7] + q * nn, sizeof(c128) * nn,
+                      hipMemcpyDeviceToDevice, c->s));
+    CK(launch_apply<double>(c->tile[0], c->op64, c->V[8], c->V[2 + q], ep, c->s));
+    if (jacobi)
+      hipLaunchKernelGGL(k_scale_diag, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, n4, c->invd,
+                         c->V[2 + q]);
+    CK(hipGetLastError());
+    g.w[q] = c->V[2 + q];
+    g.u[q] = c->V[7];
+    g.ublk[q] = q;
+  }
+  CK(hipMemcpyAsync(c->V[6], c->V[1], sizeof(c128) * n4, hipMemcpyDeviceToDevice, c->s));
+  if (jacobi)
+    hipLaunchKernelGGL(k_scale_diag, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, n4, c->invd,
+                       c->V[6]);
+  CK(hipGetLastError());
+  g.nb = 4;
+  g.b = c->V[6];
+  g.n = n4;
+  g.nn = nn;
+  return ws_fit(c, g, true, al, ok);
+}
+
+// The residual loss of the network's output `out` on the context's operator
+// and right-hand side, L = ||W (b - K x0)||^2 / ||W b||^2 (W = 1/diag if
+// jacobi, else 1) with x0 the least-squares fit of its mapped components in
+// the same norm (net_wfit), and
+// scale * dL/dout into dout ([cells][8], FP32, overwritten).  *ok is false
+// (and dout untouched) when the fit is singular or non-finite.
+int residual_out_grad(aphid_ctx* c, const float* out, bool jacobi, float* dout, double scale,
+                      double* loss, bool* ok) {
+  const long long n4 = 4 * c->nn;
+  const long long nc = (long long)(c->nx - 1) * (c->ny - 1) * (c->nz - 1);
+  std::vector<c128> al;
+  int rc = net_wfit(c, out, jacobi, al, ok);
+  if (rc || !*ok) return rc;
+  CombineArgs cb{};
+  for (int q = 0; q < 4; ++q) {
+    cb.u[q] = c->V[2 + q];
+    cb.al[q] = al[q];
+  }
+  cb.nb = 4;
+  hipLaunchKernelGGL(k_wres, dim3(kRedBlocks), dim3(kBlock), 0, c->s, n4, cb,
+                     (const c128*)c->V[6], jacobi ? (const c128*)c->invd : nullptr, c->V[8],
+                     c->part);
+  CK(hipGetLastError());
+  std::vector<double> hp(2 * (size_t)kRedBlocks);
+  CK(hipMemcpyAsync(hp.data(), c->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->s));
+  CK(hipStreamSynchronize(c->s));
+  double rr = 0.0, bb = 0.0;
+  for (int b = 0; b < kRedBlocks; ++b) rr += hp[2 * b], bb += hp[2 * b + 1];
+  if (!(bb > 0.0) || !std::isfinite(rr)) {
+    *ok = false;
+    return 0;
+  }
+  const ApplyEpilogue ep{nullptr, nullptr};
+  CK(launch_apply<double>(c->tile[0], c->op64, c->V[8], c->V[0], ep, c->s));  // z = K W conj(r)
+  hipLaunchKernelGGL(k_adj_cells, dim3(grid_for(nc)), dim3(kBlock), 0, c->s, c->nx, c->ny, c->nz,
+                     (const c128*)c->V[0], cb, 2.0 * scale / bb, dout);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(c->s));
+  *loss = rr / bb;
+  return 0;
+}
+
 }  // namespace
 
 // Initial guess for aphid_solve_resident(..., APHID_F_X0).  The candidate
@@ -1694,10 +1774,42 @@ int aphid_warm_start(aphid_ctx* c, aphid_unet* net, const float* feat, unsigned 
   if (!c->V[1]) return fail(APHID_E_STATE, "aphid_load_rhs first");
   if (net && !feat) return fail(APHID_E_ARG, "aphid_warm_start: features required with a network");
   const bool use_prev = flags & APHID_WS_PREV;
+  if ((flags & APHID_WS_JLSQ) && (!net || use_prev))
+    return fail(APHID_E_ARG, "aphid_warm_start: APHID_WS_JLSQ takes a network and no other candidate");
   CK(hipSetDevice(c->dev));
   int rc = autotune(c, 0);
   if (rc) return rc;
   const long long nn = c->nn, n4 = 4 * nn;
+  if (flags & APHID_WS_JLSQ) {
+    const float* out = nullptr;
+    int cx = 0, cy = 0, cz = 0;
+    rc = unet_internal::forward_host(net, feat, &out, &cx, &cy, &cz);
+    if (rc) return rc;
+    if (cx != c->nx - 1 || cy != c->ny - 1 || cz != c->nz - 1)
+      return fail(APHID_E_ARG, "aphid_warm_start: network grid differs from the mesh's cells");
+    std::vector<c128> al;
+    bool ok = false;
+    rc = net_wfit(c, out, true, al, &ok);
+    if (rc) return rc;
+    double rel = 1.0, rn = 0.0, bn = 0.0;
+    if (ok) {
+      WsCoef cf{};
+      cf.use_net = 1;
+      for (int k = 0; k < 4; ++k) cf.a[k] = al[k];
+      hipLaunchKernelGGL(k_ws_combine, dim3(grid_for(n4)), dim3(kBlock), 0, c->s, nn,
+                         (const c128*)c->V[7], (const c128*)c->V[6], c->V[0], cf);
+      CK(hipGetLastError());
+      rc = true_residual(c, &rel, &rn, &bn);
+      if (rc) return rc;
+    }
+    if (!ok || !std::isfinite(rel) || rel > 1.0) {  // the judge: never worse than x0 = 0
+      CK(hipMemsetAsync(c->V[0], 0, sizeof(c128) * n4, c->s));
+      CK(hipStreamSynchronize(c->s));
+      rel = 1.0;
+    }
+    if (r0_rel) *r0_rel = rel;
+    return 0;
+  }
   const ApplyEpilogue ep{nullptr, nullptr};
   GramArgs g{};
   g.b = c->V[1];
@@ -1767,6 +1879,100 @@ int aphid_warm_start(aphid_ctx* c, aphid_unet* net, const float* feat, unsigned 
     rel = 1.0;
   }
   if (r0_rel) *r0_rel = rel;
+  return 0;
+}
+
+int aphid_warm_residual_loss_grad(aphid_ctx* c, aphid_unet* net, const float* features,
+                                  unsigned flags, double* loss, float* grad) {
+  if (!c || !net || !features) return fail(APHID_E_ARG, "aphid_warm_residual_loss_grad: bad argument");
+  if (!c->have_k2 || !c->have_diag) return fail(APHID_E_STATE, "operator not set up");
+  if (!c->V[1]) return fail(APHID_E_STATE, "aphid_load_rhs first");
+  CK(hipSetDevice(c->dev));
+  int rc = autotune(c, 0);
+  if (rc) return rc;
+  const float* out = nullptr;
+  int cx = 0, cy = 0, cz = 0;
+  rc = unet_internal::forward_host(net, features, &out, &cx, &cy, &cz);
+  if (rc) return rc;
+  if (cx != c->nx - 1 || cy != c->ny - 1 || cz != c->nz - 1)
+    return fail(APHID_E_ARG, "network grid differs from the mesh's cells");
+  rc = unet_internal::zero_grads(net);
+  if (rc) return rc;
+  double L = 0.0;
+  bool ok = false;
+  rc = residual_out_grad(c, out, (flags & APHID_WS_JLSQ) != 0, unet_internal::out_grad(net), 1.0,
+                         &L, &ok);
+  if (rc) return rc;
+  if (!ok) return fail(APHID_E_STATE, "the network's guess gives a singular fit");
+  rc = unet_internal::backward_input(net, grad);
+  if (rc) return rc;
+  if (loss) *loss = L;
+  return 0;
+}
+
+int aphid_warm_train_residual(aphid_ctx* c, aphid_unet* net, const float* sigma,
+                              const double* omega, unsigned flags, int steps, int batch, float lr,
+                              unsigned long long seed, float* losses, double* seconds) {
+  if (!c || !net || !sigma || !omega || steps < 0 || batch < 1)
+    return fail(APHID_E_ARG, "aphid_warm_train_residual: bad argument");
+  if (need_mesh(c)) return fail(APHID_E_STATE, "aphid_set_mesh first");
+  if (!c->V[1]) return fail(APHID_E_STATE, "aphid_load_rhs first");
+  const int ns = unet_internal::samples(net);
+  if (ns < 1) return fail(APHID_E_STATE, "the network holds no training samples");
+  const auto t0 = std::chrono::steady_clock::now();
+  CK(hipSetDevice(c->dev));
+  const long long nc = (long long)(c->nx - 1) * (c->ny - 1) * (c->nz - 1);
+  // the operators of all samples are rebuilt on the device from their cells
+  float* dsig = nullptr;
+  CK(hipMalloc(&dsig, sizeof(float) * nc * ns));
+  struct Free {
+    float* p;
+    ~Free() { if (p) (void)hipFree(p); }
+  } guard{dsig};
+  CK(hipMemcpyAsync(dsig, sigma, sizeof(float) * nc * ns, hipMemcpyHostToDevice, c->s));
+  if (!c->sig) CK(hipMalloc(&c->sig, sizeof(c128) * nc));
+  int rc = ensure_vectors(c, false);
+  if (rc) return rc;
+  std::mt19937_64 pick(seed);
+  for (int st = 0; st < steps; ++st) {
+    rc = unet_internal::zero_grads(net);
+    if (rc) return rc;
+    double lsum = 0.0;
+    int used = 0;
+    for (int bq = 0; bq < batch; ++bq) {
+      const int idx = (int)(pick() % (unsigned long long)ns);
+      hipLaunchKernelGGL(k_sigma, dim3(grid_for(nc)), dim3(kBlock), 0, c->s, 'R', omega[idx],
+                         (const float*)(dsig + nc * idx), (const float*)nullptr,
+                         (const float*)nullptr, (const float*)nullptr, c->sig, dims(c));
+      CK(hipGetLastError());
+      rc = build_k2(c, omega[idx]);
+      if (rc) return rc;
+      if (bq == 0 && st == 0) {
+        rc = autotune(c, 0);
+        if (rc) return rc;
+      }
+      const float* out = nullptr;
+      rc = unet_internal::forward_sample(net, idx, &out);
+      if (rc) return rc;
+      double L = 0.0;
+      bool ok = false;
+      rc = residual_out_grad(c, out, (flags & APHID_WS_JLSQ) != 0, unet_internal::out_grad(net),
+                             1.0 / batch, &L, &ok);
+      if (rc) return rc;
+      if (!ok) continue;  // a singular fit contributes no gradient
+      rc = unet_internal::backward_sample(net, idx);
+      if (rc) return rc;
+      lsum += L;
+      ++used;
+    }
+    if (used) {
+      rc = unet_internal::adam_step(net, lr);
+      if (rc) return rc;
+    }
+    if (losses) losses[st] = used ? (float)(lsum / used) : NAN;
+  }
+  if (seconds)
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 

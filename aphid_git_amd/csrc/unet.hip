@@ -710,6 +710,61 @@ int aphid_unet_time(aphid_unet* n, int reps, double* fwd_seconds, double* step_s
 }  // extern "C"
 
 namespace unet_internal {
+int samples(const aphid_unet* n) { return n ? n->ds_n : 0; }
+
+int zero_grads(aphid_unet* n) {
+  UCK(hipSetDevice(n->dev));
+  const int rc = ::zero_grads(n);
+  if (rc) return rc;
+  UCK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int forward_sample(aphid_unet* n, int idx, const float** out) {
+  if (idx < 0 || idx >= n->ds_n) return aphid_fail(APHID_E_ARG, "no such training sample");
+  UCK(hipSetDevice(n->dev));
+  const bf16_t* in = n->ds_in + (size_t)n->V[0] * 8 * idx;
+  const int rc = forward(n, in);
+  if (rc) return rc;
+  UCK(hipStreamSynchronize(n->s));
+  *out = (const float*)n->buf[B_OUT];
+  return 0;
+}
+
+float* out_grad(aphid_unet* n) { return n->dbuf[B_OUT]; }
+
+int backward_sample(aphid_unet* n, int idx) {
+  if (idx < 0 || idx >= n->ds_n) return aphid_fail(APHID_E_ARG, "no such training sample");
+  UCK(hipSetDevice(n->dev));
+  const int rc = backward(n, n->ds_in + (size_t)n->V[0] * 8 * idx);
+  if (rc) return rc;
+  UCK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int backward_input(aphid_unet* n, float* grad) {
+  UCK(hipSetDevice(n->dev));
+  const int rc = backward(n, nullptr);
+  if (rc) return rc;
+  if (grad) UCK(hipMemcpyAsync(grad, n->g, sizeof(float) * n->nparam, hipMemcpyDeviceToHost, n->s));
+  UCK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int adam_step(aphid_unet* n, float lr) {
+  UCK(hipSetDevice(n->dev));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  ++n->adam_t;
+  const float bc1 = 1.f - std::pow(b1, (float)n->adam_t), bc2 = 1.f - std::pow(b2, (float)n->adam_t);
+  hipLaunchKernelGGL(kc_adam, dim3(grid_for(n->nparam)), dim3(kBlock), 0, n->s, n->nparam, n->p,
+                     (const float*)n->g, n->m, n->v, lr, b1, b2, eps, bc1, bc2, 1.f);
+  UCK(hipGetLastError());
+  const int rc = refresh_weights(n);
+  if (rc) return rc;
+  UCK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
 int forward_host(aphid_unet* n, const float* feat, const float** out, int* nx, int* ny, int* nz) {
   int rc = aphid_unet_forward(n, feat, nullptr);
   if (rc) return rc;

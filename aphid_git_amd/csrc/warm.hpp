@@ -104,6 +104,80 @@ __global__ void k_combine(long long n, CombineArgs g, c128* __restrict__ x) {
   }
 }
 
+// x[q] = d[q] * x[q] (the Jacobi weight W = M^-1 = 1/diag)
+__global__ void k_scale_diag(long long n, const c128* __restrict__ d, c128* __restrict__ x) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    x[q] = d[q] * x[q];
+}
+
+// Residual of the fitted guess in the W-weighted norm (W = 1/diag, or the
+// identity when invd is null):
+//   r = W b - sum_c al_c W K P_c u  (w[c] = W K P_c u, bw = W b),
+// y = W conj(r) (the input of the adjoint apply), and per-block partials of
+// ||r||^2 and ||W b||^2.
+__global__ __launch_bounds__(256) void k_wres(long long n, CombineArgs g,
+                                              const c128* __restrict__ bw,
+                                              const c128* __restrict__ invd,
+                                              c128* __restrict__ y, double* part) {
+  double acc[2] = {0.0, 0.0};
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const c128 b = bw[q];
+    c128 r = b;
+    for (int a = 0; a < g.nb; ++a) {
+      const c128 t = g.al[a] * g.u[a][q];
+      r = {r.re - t.re, r.im - t.im};
+    }
+    y[q] = invd ? invd[q] * c128{r.re, -r.im} : c128{r.re, -r.im};
+    acc[0] += abs2(r);
+    acc[1] += abs2(b);
+  }
+  block_sum<2>(acc);
+  block_partial<2>(acc, part);
+}
+
+// d loss / d output of the network for the loss L = ||r||^2 / ||W b||^2 of
+// the fitted guess x0 = sum_c al_c P_c u, u = the cells mapped onto the
+// unknowns (k_cells_to_unknowns).  With z = K W conj(r) (K complex
+// symmetric), dL = -(2/||Wb||^2) Re(sum_q al_c(q) z_q du_q), so
+//   dL/dRe u_q = -s Re(al_c z_q),  dL/dIm u_q = s Im(al_c z_q),
+// s = 2 / (||Wb||^2 batch).  Each cell gathers the unknowns the mapping
+// fed from it (its lower corner node; on the last cell planes also the
+// nodes beyond).  dout[cell][2c], dout[cell][2c+1] are overwritten.
+__global__ void k_adj_cells(int nx, int ny, int nz, const c128* __restrict__ z, CombineArgs g,
+                            double s, float* __restrict__ dout) {
+  const long long nn = (long long)nx * ny * nz;
+  const int cx = nx - 1, cy = ny - 1, cz = nz - 1;
+  const long long ncell = (long long)cx * cy * cz;
+  for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < ncell;
+       cell += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(cell % cz), j = (int)((cell / cz) % cy), i = (int)(cell / ((long long)cz * cy));
+    double gr[4] = {0.0, 0.0, 0.0, 0.0}, gi[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int di = 0; di <= (i == cx - 1 ? 1 : 0); ++di)
+      for (int dj = 0; dj <= (j == cy - 1 ? 1 : 0); ++dj)
+        for (int dk = 0; dk <= (k == cz - 1 ? 1 : 0); ++dk) {
+          const int ii = i + di, jj = j + dj, kk = k + dk;
+          const long long node = ((long long)ii * ny + jj) * nz + kk;
+          for (int c = 0; c < 4; ++c) {
+            bool valid;
+            if (c == 3)
+              valid = ii > 0 && ii < nx - 1 && jj > 0 && jj < ny - 1 && kk > 0 && kk < nz - 1;
+            else
+              valid = (c == 0 ? ii : c == 1 ? jj : kk) < (c == 0 ? cx : c == 1 ? cy : cz);
+            if (!valid) continue;
+            const c128 t = g.al[c] * z[c * nn + node];
+            gr[c] -= s * t.re;
+            gi[c] += s * t.im;
+          }
+        }
+    for (int c = 0; c < 4; ++c) {
+      dout[cell * 8 + 2 * c] = (float)gr[c];
+      dout[cell * 8 + 2 * c + 1] = (float)gi[c];
+    }
+  }
+}
+
 struct WsCoef {
   c128 a[4];  // amplitude of each component of the mapped prediction
   c128 prev;  // amplitude of the previous solution

@@ -33,6 +33,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20
 TILES = [(16, 16), (32, 8), (64, 4), (16, 8), (32, 4), (32, 16), (64, 8), (16, 32)]
 WS_PREV = 1
 WS_LSQ = 2
+WS_JLSQ = 4
 STATUS = {0: "converged", 1: "max_it", 2: "breakdown"}
 
 
@@ -268,14 +269,15 @@ class Solver:
                            hist[:rep.hist_len].copy())
 
     def warm_start(self, net: Optional["UNet"] = None, features=None, prev: bool = False,
-                   lsq: bool = False) -> float:
+                   lsq: bool = False, jlsq: bool = False) -> float:
         """Initial guess for solve_resident(..., flags=F_X0) from the network
         and/or the current solution (prev), amplitudes by the Galerkin
-        condition (or least squares on the residual); returns
+        condition (or least squares on the residual; jlsq: least squares on
+        the Jacobi-weighted residual, network only); returns
         ||b - K x0|| / ||b||."""
         r0 = C.c_double()
         f = None if features is None else np.ascontiguousarray(features, np.float32)
-        flags = (WS_PREV if prev else 0) | (WS_LSQ if lsq else 0)
+        flags = (WS_PREV if prev else 0) | (WS_LSQ if lsq else 0) | (WS_JLSQ if jlsq else 0)
         _check(lib().aphid_warm_start(self._ctx, None if net is None else C.c_void_p(net._n),
                                       _p(f), C.c_uint(flags), C.byref(r0)), "aphid_warm_start")
         return r0.value
@@ -304,6 +306,37 @@ class Solver:
         return out
 
     # -- tuning / timing ------------------------------------------------------
+    def residual_loss_grad(self, net: "UNet", features, jacobi: bool = True):
+        """L = ||W (b - K x0)||^2 / ||W b||^2 of the network's fitted guess
+        on this context's operator and resident b (W = 1/diag if jacobi,
+        else 1), and dL/dparams."""
+        g = np.zeros(net.n_params, np.float32)
+        loss = C.c_double()
+        _check(lib().aphid_warm_residual_loss_grad(self._ctx, C.c_void_p(net._n),
+                                                   _p(net._f(features, 3)),
+                                                   C.c_uint(WS_JLSQ if jacobi else WS_LSQ),
+                                                   C.byref(loss), _p(g)),
+               "aphid_warm_residual_loss_grad")
+        return loss.value, g
+
+    def train_residual(self, net: "UNet", sigmas, omegas, steps: int, batch: int = 4,
+                       lr: float = 1e-3, seed: int = 0, jacobi: bool = True):
+        """Residual training over the network's stored samples: sigmas
+        [nsamples][cells] (model R, Fortran order per sample), omegas
+        [nsamples]; this context holds the mesh and the shared b.  Returns
+        (mean loss per step, seconds)."""
+        sig = np.ascontiguousarray(
+            np.stack([np.asarray(s, np.float32).ravel(order="F") for s in sigmas]), np.float32)
+        om = np.ascontiguousarray(omegas, np.float64)
+        losses = np.zeros(max(steps, 1), np.float32)
+        secs = C.c_double()
+        _check(lib().aphid_warm_train_residual(self._ctx, C.c_void_p(net._n), _p(sig), _p(om),
+                                               C.c_uint(WS_JLSQ if jacobi else WS_LSQ),
+                                               steps, batch, C.c_float(lr), C.c_ulonglong(seed),
+                                               _p(losses), C.byref(secs)),
+               "aphid_warm_train_residual")
+        return losses[:steps], secs.value
+
     def get_tile(self, precision: int = 0):
         t = [C.c_int(), C.c_int(), C.c_int()]
         _check(lib().aphid_get_tile(self._ctx, precision, *[C.byref(v) for v in t]), "aphid_get_tile")
@@ -531,4 +564,5 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_unet_forward_flops", "aphid_unet_get_params", "aphid_unet_set_params",
            "aphid_unet_forward", "aphid_unet_loss_grad", "aphid_unet_activation",
            "aphid_unet_reserve", "aphid_unet_add_sample", "aphid_unet_train", "aphid_unet_time",
-           "aphid_warm_start", "aphid_gen_fields", "aphid_get_sigma"]
+           "aphid_warm_start", "aphid_warm_residual_loss_grad", "aphid_warm_train_residual",
+           "aphid_gen_fields", "aphid_get_sigma"]

@@ -141,6 +141,112 @@ def evaluate(net: Optional[gpu.UNet], ncell, seeds: Sequence[int], tol: float = 
     return out
 
 
+def protocol(cells, ntrain: int, nhold: int, mse_steps: int, res_steps: int, batch: int = 4,
+             lr: float = 2e-3, tol: float = 1e-8, max_it: int = 20000, seed0: int = 1000,
+             hold0: int = 500000, log=None) -> dict:
+    """The learned warm start end to end on random-block models of one cell
+    grid (frequencies log-uniform in 1e-2..1e5 Hz, the same surface dipole):
+
+    1. ntrain solver-made training samples: each model solved from x0 = 0
+       to tol, its features and normalised solution stored on the GPU;
+    2. two networks trained on the GPU over those models:
+       "field": mse_steps Adam steps on the field (MSE against the solver's
+       solutions), the guess fitted by the Galerkin condition as before;
+       "residual": res_steps on the residual of the network's own guess,
+       ||b - K x0||^2 / ||b||^2 after the least-squares fit of its four
+       component amplitudes (aphid_warm_train_residual), fitted the same way;
+    3. nhold held-out models, each solved from x0 = 0 and from each
+       network's guess (the true residual as the judge: never worse than
+       x0 = 0): iterations and seconds to tol.
+    """
+    t_all = time.time()
+    s = gpu.Solver(0)
+    nets = {"field": gpu.UNet(cells, seed=0), "residual": gpu.UNet(cells, seed=1)}
+    for n in nets.values():
+        n.reserve(ntrain)
+    zeros = np.zeros((int(np.prod(cells)), 8), np.float32)
+    sigmas, omegas, its = [], [], []
+    b0 = None
+    t0 = time.time()
+    for q in range(ntrain):
+        case = random_case(cells, seed0 + q)
+        r, _, feat, b = solve_case(s, case, tol, max_it)
+        if b0 is None:
+            b0 = b
+        nets["field"].add_sample(feat, target(s.fetch_solution()))
+        nets["residual"].add_sample(feat, zeros)
+        sigmas.append(np.asarray(case.cells["sigma"], np.float32))
+        omegas.append(case.omega)
+        its.append(r.iters)
+        if log and (q + 1) % max(1, ntrain // 10) == 0:
+            log(f"[protocol] data {q + 1}/{ntrain} mean iters {np.mean(its):.0f} "
+                f"{time.time() - t0:.0f}s")
+    t_data = time.time() - t0
+    train = {}
+    l_f, t_f = nets["field"].train(mse_steps, batch=batch, lr=lr, seed=1)
+    train["field"] = {"steps": mse_steps, "seconds": t_f, "loss_first": float(l_f[:5].mean()),
+                      "loss_last": float(l_f[-20:].mean())}
+    s.setup_case(random_case(cells, seed0))  # the mesh; b is the same dipole for every case
+    s.load_rhs(b0)
+    l_r, t_r = s.train_residual(nets["residual"], sigmas, omegas, res_steps, batch=batch, lr=lr,
+                                seed=2, jacobi=False)
+    train["residual"] = {"steps": res_steps, "seconds": t_r, "loss_first": float(l_r[:5].mean()),
+                         "loss_last": float(l_r[-20:].mean())}
+    if log:
+        log(f"[protocol] trained: {train}")
+    del sigmas
+    ev = []
+    for q in range(nhold):
+        case = random_case(cells, hold0 + q)
+        rz, _, feat, b = solve_case(s, case, tol, max_it)
+        e = {"freq": case.freq, "iters_zero": rz.iters, "sec_zero": rz.seconds_total,
+             "ok_zero": rz.status == 0 and rz.true_rel_res <= tol}
+        for name, net in nets.items():
+            s.load_rhs(b)
+            t = time.perf_counter()
+            r0 = s.warm_start(net, feat, lsq=(name == "residual"))
+            t_ws = time.perf_counter() - t
+            rw = s.solve_resident(tol, max_it, gpu.F_X0)
+            e[name] = {"iters": rw.iters, "sec": rw.seconds_total + t_ws, "r0": r0,
+                       "ok": rw.status == 0 and rw.true_rel_res <= tol}
+        ev.append(e)
+        if log and (q + 1) % max(1, nhold // 8) == 0:
+            log(f"[protocol] held-out {q + 1}/{nhold}")
+    s.close()
+    for n in nets.values():
+        n.close()
+    iz = float(np.mean([e["iters_zero"] for e in ev]))
+    sz = float(np.mean([e["sec_zero"] for e in ev]))
+    out = {"cells": list(cells), "train_samples": ntrain, "held_out": nhold,
+           "data_seconds": t_data, "data_iters_mean": float(np.mean(its)),
+           "iters_zero_mean": iz, "seconds_zero_mean": sz,
+           "all_zero_converged": all(e["ok_zero"] for e in ev)}
+    for name in nets:
+        iw = float(np.mean([e[name]["iters"] for e in ev]))
+        sw = float(np.mean([e[name]["sec"] for e in ev]))
+        out[name] = {**train[name], "iters_warm_mean": iw, "iters_ratio": iw / iz,
+                     "seconds_warm_mean": sw, "seconds_ratio": sw / sz,
+                     "r0_median": float(np.median([e[name]["r0"] for e in ev])),
+                     "r0_min": float(np.min([e[name]["r0"] for e in ev])),
+                     "all_warm_converged": all(e[name]["ok"] for e in ev),
+                     "per_decade": _per_decade(ev, name)}
+    out["seconds_total"] = time.time() - t_all
+    return out
+
+
+def _per_decade(ev, name):
+    out = {}
+    for e in ev:
+        d = int(np.floor(np.log10(e["freq"])))
+        o = out.setdefault(f"1e{d}", {"n": 0, "iters_zero": 0, "iters_warm": 0})
+        o["n"] += 1
+        o["iters_zero"] += e["iters_zero"]
+        o["iters_warm"] += e[name]["iters"]
+    for o in out.values():
+        o["ratio"] = o["iters_warm"] / o["iters_zero"]
+    return out
+
+
 def cells_to_unknowns(out: np.ndarray, nodes) -> List[np.ndarray]:
     """Host restatement of the solver's mapping of a cell-grid prediction
     [V][8] onto the staggered unknowns (reference shapes): each unknown

@@ -22,9 +22,11 @@ Further parts (--parts, default all), reported as extra JSON fields:
   unet   the warm-start network on a 96x96x80 cell grid: forward and
          forward+backward time, TFLOP/s against the 2.5 PFLOP/s dense bf16
          matrix-core peak;
-  warm   a small learned warm start end to end (16^3 cells): solver-made
-         training set, training on the GPU, iterations vs x0 = 0 on
-         held-out models.
+  warm   the learned warm start's protocol on 16^3 cells: solver-made
+         training set, two networks trained on the GPU (field MSE; the
+         residual of the network's own guess), iterations vs x0 = 0 on
+         held-out models.  --parts warm5 runs it at 96x96x80 cells with
+         2000 training and 256 held-out models (about 15 minutes).
 """
 from __future__ import annotations
 
@@ -53,7 +55,7 @@ BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
 PARTS = ("solve", "config4", "shard", "batch", "unet", "warm", "cpu")
-EXTRA_PARTS = ("batch3b", "warm5", "prec4")  # not in the default set (minutes of GPU time)
+EXTRA_PARTS = ("batch3b", "warm5", "prec4", "prec4sweep")  # not default (minutes of GPU time)
 
 
 def log(msg):
@@ -287,16 +289,26 @@ def bench_batch3b(max_it: int = 10000):
                               for q, n in enumerate((64, 96, 128))}}
 
 
-def bench_warm5(ntrain=128, steps=1500, neval=16):
-    """The learned warm start at its full cell grid (96x96x80 cells)."""
-    return bench_warm(ncell=(96, 96, 80), ntrain=ntrain, steps=steps, neval=neval)
+def bench_warm5(ntrain=2000, nhold=256, mse_steps=2000, res_steps=1500):
+    """The learned warm start's protocol at its cell grid (96x96x80 cells):
+    2000 solver-made training solutions, two networks trained on the GPU
+    (field MSE against them; the residual of the network's own guess), 256
+    held-out models solved from x0 = 0 and from each network's guess.
+    About 15 minutes."""
+    from aphid_git_amd import warmstart
+
+    out = warmstart.protocol((96, 96, 80), ntrain, nhold, mse_steps, res_steps, log=log)
+    out["workload"] = ("random-block models on 96x96x80 cells (97x97x81 nodes), frequencies "
+                       "log-uniform 1e-2..1e5 Hz, +X surface dipole, rel-res 1e-8")
+    return out
 
 
-def bench_prec4(freq: float = 1e4, max_it: int = 100000, stall: int = 1000):
+def bench_prec4(freq: float = 1e4, max_it: int = 300000, max_it_mixed: int = 100000,
+                stall: int = 1000):
     """Config 4 (192x192x160 cells, 23.9M unknowns, contrast 1e5) at one
-    frequency: the complex128 solve against mixed precision (complex64
-    stencil and vectors, FP64 scalars and refinement; inner solves stop on
-    stagnation) with the same iteration cap."""
+    frequency: the complex128 solve to rel-res 1e-8 (capped at max_it) and
+    mixed precision (complex64 stencil and vectors, FP64 scalars and
+    refinement; inner solves stop on stagnation) capped at max_it_mixed."""
     from aphid_git_amd import gpu, meshgen
 
     os.environ.setdefault("APHID_PROGRESS", "5000")  # read once, by the first solve
@@ -305,13 +317,14 @@ def bench_prec4(freq: float = 1e4, max_it: int = 100000, stall: int = 1000):
     b = s.setup_case(case)
     out = {"workload": f"config4: 192x192x160-cell stretched mesh, 40 blocks, log10 sigma in [-5, 0], "
                        f"+X dipole, {freq:g} Hz, rel-res 1e-8", "unknowns": case.n_unknowns,
-           "max_it": max_it}
+           "max_it": max_it, "max_it_mixed": max_it_mixed}
     s.load_rhs(b)
     t0 = time.perf_counter()
     r = s.solve_resident(TOL, max_it)
     out["c128"] = {"seconds": time.perf_counter() - t0, "iters": r.iters, "status": r.status,
                    "true_rel_res": r.true_rel_res}
     log(f"[bench] prec4 c128: {out['c128']}")
+    max_it = max_it_mixed
     x128 = np.concatenate([a.ravel() for a in s.fetch_solution()])
     os.environ["APHID_MIXED_STALL"] = str(stall)
     s.load_rhs(b)
@@ -326,6 +339,46 @@ def bench_prec4(freq: float = 1e4, max_it: int = 100000, stall: int = 1000):
     log(f"[bench] prec4 mixed: {out['mixed']}")
     s.close()
     return out
+
+
+def bench_prec4_sweep(cap: int = 10000, stall: int = 1000):
+    """Config 4 at its 8 frequencies (log-spaced 1e-1..1e4 Hz) with a fixed
+    budget of `cap` iterations each: the complex128 solve and mixed
+    precision (complex64 inner solves capped at the same total, stagnation
+    window `stall`, FP64 refinement), their true residuals, seconds, and
+    the field difference between the two."""
+    from aphid_git_amd import gpu, meshgen
+
+    s = gpu.Solver(0)
+    rows = []
+    for f in np.logspace(-1, 4, 8):
+        case = meshgen.config4(freq=float(f))
+        b = s.setup_case(case)
+        s.load_rhs(b)
+        t0 = time.perf_counter()
+        r = s.solve_resident(TOL, cap)
+        t128 = time.perf_counter() - t0
+        x128 = np.concatenate([a.ravel() for a in s.fetch_solution()])
+        os.environ["APHID_MIXED_STALL"] = str(stall)
+        s.load_rhs(b)
+        t0 = time.perf_counter()
+        rm = s.solve_resident(TOL, cap, gpu.F_MIXED)
+        tm = time.perf_counter() - t0
+        del os.environ["APHID_MIXED_STALL"]
+        xm = np.concatenate([a.ravel() for a in s.fetch_solution()])
+        row = {"freq": float(f),
+               "c128": {"iters": r.iters, "status": r.status, "true_rel_res": r.true_rel_res,
+                        "seconds": t128, "ms_per_iteration": 1e3 * t128 / max(min(r.iters, cap), 1)},
+               "mixed": {"iters_c64": rm.iters, "sweeps": rm.outer_iters, "status": rm.status,
+                         "true_rel_res": rm.true_rel_res, "seconds": tm,
+                         "ms_per_iteration": 1e3 * tm / max(min(rm.iters, cap), 1)},
+               "field_rel_diff": float(np.linalg.norm(xm - x128) / np.linalg.norm(x128))}
+        rows.append(row)
+        log(f"[bench] prec4 sweep {f:.3g} Hz: {row}")
+    s.close()
+    return {"workload": "config4: 192x192x160-cell stretched mesh, 40 blocks, log10 sigma in "
+                        "[-5, 0], +X dipole, 8 frequencies log-spaced 1e-1..1e4 Hz",
+            "iteration_cap": cap, "stall_window": stall, "frequencies": rows}
 
 
 def bench_config4():
@@ -411,25 +464,15 @@ def bench_unet(reps: int = 5):
                          "unit": "TFLOP/s", "frac": F / f / 1e12 / BF16_PEAK_TFLOPS}}
 
 
-def bench_warm(ncell=(16, 16, 16), ntrain=64, steps=400, neval=8):
-    from aphid_git_amd import gpu, warmstart
+def bench_warm(ncell=(16, 16, 16), ntrain=64, nhold=16, mse_steps=400, res_steps=400):
+    """The learned warm start's protocol on a small grid (seconds): what the
+    trained network gains over x0 = 0 on held-out models."""
+    from aphid_git_amd import warmstart
 
-    net = gpu.UNet(ncell, seed=0)
-    net.reserve(ntrain)
-    data = warmstart.build_dataset(net, ncell, range(1000, 1000 + ntrain))
-    losses, t_train = net.train(steps, batch=4, lr=2e-3, seed=1)
-    ev = warmstart.evaluate(net, ncell, range(5000, 5000 + neval))
-    net.close()
-    iz = sum(r["iters_zero"] for r in ev)
-    iw = sum(r["iters_warm"] for r in ev)
-    return {"workload": f"random-block models on {ncell[0]}x{ncell[1]}x{ncell[2]} cells, "
-                        f"frequencies log-uniform 1e-2..1e5 Hz, +X surface dipole",
-            "train_samples": ntrain, "data_seconds": data["seconds"], "train_steps": steps,
-            "train_seconds": t_train, "loss_first": float(losses[:5].mean()),
-            "loss_last": float(losses[-20:].mean()), "eval_systems": neval,
-            "iters_zero_mean": iz / neval, "iters_warm_mean": iw / neval, "iters_ratio": iw / iz,
-            "all_warm_converged": all(r["status_warm"] == 0 and r["true_warm"] <= TOL for r in ev),
-            "r0_median": float(np.median([r["r0"] for r in ev]))}
+    out = warmstart.protocol(ncell, ntrain, nhold, mse_steps, res_steps)
+    out["workload"] = (f"random-block models on {ncell[0]}x{ncell[1]}x{ncell[2]} cells, "
+                       f"frequencies log-uniform 1e-2..1e5 Hz, +X surface dipole, rel-res 1e-8")
+    return out
 
 
 def main():
@@ -596,6 +639,9 @@ def main():
     if "prec4" in parts:
         log("[bench] precision study on config 4 ...")
         out["precision_config4"] = bench_prec4()
+    if "prec4sweep" in parts:
+        log("[bench] precision study on config 4, 8 frequencies ...")
+        out["precision_config4_sweep"] = bench_prec4_sweep()
     if "warm5" in parts:
         log("[bench] learned warm start, 96x96x80 cells ...")
         out["warm_start_96x96x80"] = bench_warm5()

@@ -327,6 +327,11 @@ int aphid_unet_time(aphid_unet *net, int reps, double *fwd_seconds,
 
 #define APHID_WS_PREV 1u /* include the context's current solution */
 #define APHID_WS_LSQ 2u  /* fit amplitudes by least squares on the residual */
+#define APHID_WS_JLSQ 4u /* network only: amplitudes by least squares on the
+                            Jacobi-weighted residual ||W (b - K x0)||,
+                            W = 1/diag, the fit aphid_warm_train_residual
+                            trains with; x0 = 0 if it leaves
+                            ||b - K x0|| > ||b|| */
 /* Initial guess for aphid_solve_resident(..., APHID_F_X0) from the network
  * (net may be NULL) and/or the context's current solution.  One complex
  * amplitude per candidate (each network component, the previous solution)
@@ -338,6 +343,32 @@ int aphid_unet_time(aphid_unet *net, int reps, double *fwd_seconds,
  * r0_rel (optional): ||b - K x0|| / ||b||. */
 int aphid_warm_start(aphid_ctx *ctx, aphid_unet *net, const float *features,
                      unsigned flags, double *r0_rel);
+
+/* The residual loss below for one case -- the context's operator and
+ * resident right-hand side, the network's features -- and its gradient
+ * with respect to the parameters (grad, optional, FP32 [param count]); the
+ * network's output gradient stays readable with aphid_unet_activation. */
+int aphid_warm_residual_loss_grad(aphid_ctx *ctx, aphid_unet *net, const float *features,
+                                  unsigned flags, double *loss, float *grad);
+
+/* Train the network on the residual of its own guess (no counterpart in
+ * the reference).  Each step draws `batch` of the network's stored samples
+ * (aphid_unet_add_sample, in the order of sigma and omega), rebuilds that
+ * sample's operator on the device from its cells (model R: sigma
+ * [nsamples][(nx-1)(ny-1)(nz-1)], Fortran order per sample; angular
+ * frequency omega[sample]), fits the four component amplitudes of the
+ * mapped output by least squares on the residual, and back-propagates
+ *   L = ||W (b - K x0)||^2 / ||W b||^2   (b the context's resident
+ * right-hand side, shared by all samples; flags APHID_WS_JLSQ: W = 1/diag,
+ * the fit of aphid_warm_start's APHID_WS_JLSQ; else W = 1, the fit of its
+ * APHID_WS_LSQ for a network alone), whose gradient needs one
+ * more operator apply (K is complex symmetric: K^H y = conj(K conj y)); then
+ * one Adam step.  losses (optional): mean L per step.  Overwrites the
+ * context's operator and resident vectors (reload b before solving). */
+int aphid_warm_train_residual(aphid_ctx *ctx, aphid_unet *net, const float *sigma,
+                              const double *omega, unsigned flags, int steps, int batch,
+                              float lr, unsigned long long seed, float *losses,
+                              double *seconds);
 
 #ifdef __cplusplus
 }

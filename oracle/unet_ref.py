@@ -156,6 +156,17 @@ def loss_grad(flat, feat, target, cells):
     return float(loss), g.numpy()
 
 
+def backward_from(flat, feat, cells, dout):
+    """Parameter gradient of sum(out * dout) for a given output gradient
+    dout [V][8]: the network's backward pass from an arbitrary d loss /
+    d output (as a loss computed outside the network produces)."""
+    P = split_params(flat, requires_grad=True)
+    out = forward(P, feat, cells)
+    d = torch.as_tensor(np.asarray(dout, np.float64))
+    (to_voxels_t(out) * d).sum().backward()
+    return torch.cat([torch.cat([w.grad.reshape(-1), b.grad]) for w, b in P]).numpy()
+
+
 def to_voxels_t(t):
     t = t[0]
     return t.permute(1, 2, 3, 0).reshape(-1, t.shape[0])

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from aphid_git_amd import meshgen
-from conftest import flat4, golden_files, load_golden, relerr, shapes, split4
+from conftest import GOLDEN, flat4, golden_files, load_golden, relerr, shapes, split4
 
 pytestmark = pytest.mark.gpu
 
@@ -205,13 +205,22 @@ def test_zero_rhs_and_breakdown_are_statuses(gpu):
     assert r.status == 0 and r.iters == 0
 
 
+def _config2_golden():
+    path = os.path.join(GOLDEN, "config2_1Hz.npz")
+    if not os.path.exists(path):
+        pytest.skip("tests/golden/config2_1Hz.npz not generated yet")
+    return dict(np.load(path))
+
+
 def test_full_size_config2_operator_symmetry_and_solve(gpu):
-    """128x128x96 cells (the benchmark mesh): size-independent checks -- K is
-    complex symmetric on the interior-Phi subspace (u^T K v = v^T K u), the
-    first QMR residuals equal the reference Fortran's own qmr.iter on this
-    system (iterations 10, 20, 30, printed to 9 digits by a reference run),
-    and the solve's true residual meets rel-res 1e-8 (about 47k iterations:
-    the 1.3x padding out to ~480 km makes this system converge slowly)."""
+    """128x128x96 cells (the benchmark mesh): K is complex symmetric on the
+    interior-Phi subspace (u^T K v = v^T K u); the first 30 QMR residuals
+    equal the reference Fortran's own qmr.iter on this system (its serial
+    qmr, 30 iterations, stored in tests/golden/config2_1Hz.npz by
+    tests/golden/make_config2_golden.py); and the solve meets rel-res 1e-8
+    by its true residual (about 46k iterations: the 1.3x padding out to
+    ~480 km makes this system converge slowly).  The converged fields are
+    compared with the oracle's in the next test."""
     case = meshgen.config2()
     print(f"[config2] {case.shape} M={case.n_unknowns}", flush=True)
     b = gpu.setup_case(case)
@@ -223,14 +232,38 @@ def test_full_size_config2_operator_symmetry_and_solve(gpu):
     Ku, Kv = flat4(gpu.apply(u)), flat4(gpu.apply(v))
     a, bb = flat4(v) @ Ku, flat4(u) @ Kv
     assert abs(a - bb) <= 1e-11 * np.linalg.norm(flat4(u)) * np.linalg.norm(Kv)
+    z = _config2_golden()
     r = gpu.qmr_solve(b, 1e-8, 30)
-    np.testing.assert_allclose(r.hist[[9, 19, 29]],
-                               [1.41784782e-06, 5.70217783e-07, 3.34779008e-07], rtol=1e-8)
+    np.testing.assert_allclose(r.hist[:30], z["ref_hist30"], rtol=1e-8)
+
+
+def test_config2_fields_match_the_oracle(gpu):
+    """The benchmark system solved to rel-res 1e-8 on the GPU against the
+    oracle's 4-thread complex128 solve of the same system (which follows
+    the reference's arithmetic order; tests/golden/config2_1Hz.npz holds
+    its iteration count, residual history and a seeded 1% sample of its
+    solution).  Both solves run the same Krylov process from x0 = 0, but
+    their reductions round differently, and on this system's long plateau
+    (residual near 2e-3 from ~500 to ~15,000 iterations) the last bits
+    steer the iterates apart; the two rel-res-1e-8 solutions then differ
+    by what that tolerance leaves of the field error.  The test states the
+    measured gap: sampled fields within 1e-6 (relative 2-norm), component
+    norms within 1e-6, iteration counts within 10%."""
+    case = meshgen.config2()
+    z = _config2_golden()
+    b = gpu.setup_case(case)
     r = gpu.qmr_solve(b, 1e-8, 100000, hist_cap=0)
-    print(f"[config2] iters {r.iters} status {r.status} true {r.true_rel_res:.3e} "
-          f"t {r.seconds_qmr:.3f}s", flush=True)
-    assert r.status == 0
-    assert r.true_rel_res < 1.05e-8
+    assert r.status == 0 and r.true_rel_res < 1.05e-8
+    sol = flat4(r.x)
+    idx, val = z["sol_idx"].astype(np.int64), z["sol_val"]
+    err = np.linalg.norm(sol[idx] - val) / np.linalg.norm(val)
+    norms = np.array([np.linalg.norm(a) for a in r.x])
+    nerr = np.max(np.abs(norms - z["oracle_comp_norms"]) / z["oracle_comp_norms"])
+    print(f"[config2] GPU {r.iters} iterations (oracle {int(z['oracle_iters'])}), true rel res "
+          f"{r.true_rel_res:.3e} (oracle {float(z['oracle_true_rel_res']):.3e}); sampled field "
+          f"difference {err:.3e}, component norms {nerr:.3e}", flush=True)
+    assert abs(r.iters - int(z["oracle_iters"])) <= 0.1 * int(z["oracle_iters"])
+    assert err < 1e-6 and nerr < 1e-6
 
 
 # --------------------------------------------------------------------------

@@ -203,3 +203,129 @@ def test_warm_start_maps_the_prediction_and_fits_galerkin_amplitudes(cells):
     assert warm.status == 0 and warm.iters <= 1  # the loop tests after its first update
     s.close()
     net.close()
+
+
+def _residual_loss_host(out, case, b, S):
+    """Host float64 restatement of the residual loss of a network output:
+    map the cells onto the unknowns, fit the four component amplitudes by
+    least squares on the Jacobi-weighted residual, L = ||r||^2 / ||W b||^2
+    (the oracle's operator and diagonal)."""
+    from aphid_git_amd import warmstart
+
+    u = warmstart.cells_to_unknowns(out, case.shape)
+    W = np.concatenate([1.0 / a.ravel(order="F") for a in S.diag])
+    cols = []
+    for c in range(4):
+        uc = [np.zeros_like(a) for a in u]
+        uc[c] = u[c]
+        cols.append(W * np.concatenate([a.ravel(order="F") for a in S.op.apply(uc)]))
+    A = np.stack(cols, 1)
+    bw = W * np.concatenate([a.ravel(order="F") for a in b])
+    al = np.linalg.lstsq(A, bw, rcond=None)[0]
+    r = bw - A @ al
+    return float(np.vdot(r, r).real / np.vdot(bw, bw).real)
+
+
+def test_residual_loss_and_gradient(oracle):
+    """The residual loss L = ||W (b - K x0)||^2 / ||W b||^2 of the network's
+    fitted guess (W = 1/diag): L equals the host float64 restatement on the
+    oracle's operator (1e-8); d L / d output equals central finite
+    differences of that host loss along random directions (1e-4); and the
+    parameter gradient equals the CPU reference network's backward pass from
+    the GPU's own output gradient (3e-2 per layer, cosine > 0.999)."""
+    import unet_ref
+    from aphid_git_amd import gpu, meshgen, warmstart
+
+    case = meshgen.random_block_case(CELLS, 13, 20.0)
+    s = gpu.Solver(0)
+    b = s.setup_case(case)
+    s.load_rhs(b)
+    feat = warmstart.features(case, b)
+    net = _net(14)
+    L, g = s.residual_loss_grad(net, feat)
+    out = net.activation("out").astype(np.float64)
+    dout = net.activation("out", grad=True).astype(np.float64)
+    S = oracle.System(case)
+    Lh = _residual_loss_host(out, case, b, S)
+    print(f"[residual loss] GPU {L:.12e} host {Lh:.12e}", flush=True)
+    assert 0.0 < L < 1.0 and abs(L - Lh) <= 1e-8 * Lh
+    rng = np.random.default_rng(15)
+    for _ in range(3):
+        d = rng.standard_normal(out.shape)
+        h = 1e-4 * np.linalg.norm(out) / np.linalg.norm(d)
+        fd = (_residual_loss_host(out + h * d, case, b, S) -
+              _residual_loss_host(out - h * d, case, b, S)) / (2 * h)
+        an = float(np.sum(dout * d))
+        print(f"[residual loss] directional derivative FD {fd:.8e} analytic {an:.8e}", flush=True)
+        assert abs(fd - an) <= 1e-4 * max(abs(fd), 1e-3 * np.linalg.norm(dout) * np.linalg.norm(d))
+    gref = unet_ref.backward_from(net.get_params(), feat, CELLS, dout)
+    off = 0
+    for l, (k, ci, co) in enumerate(unet_ref.LAYERS):
+        nw = co * ci * k ** 3
+        for name, sl in (("w", slice(off, off + nw)), ("b", slice(off + nw, off + nw + co))):
+            err = np.linalg.norm(g[sl] - gref[sl]) / max(np.linalg.norm(gref[sl]), 1e-30)
+            assert err < 3e-2, (l, name, err)
+        off += nw + co
+    cos = float(g @ gref) / (np.linalg.norm(g) * np.linalg.norm(gref))
+    assert cos > 0.999, cos
+    s.close()
+    net.close()
+
+
+def test_residual_training_reduces_the_residual_of_the_guess():
+    """A few hundred residual-training steps on solver-made samples of one
+    small model family lower the mean residual loss, and the trained guess
+    (APHID_WS_JLSQ) never starts worse than x0 = 0."""
+    from aphid_git_amd import gpu, warmstart
+
+    cells = (16, 16, 16)
+    net = gpu.UNet(cells, seed=21)
+    s = gpu.Solver(0)
+    cases = [warmstart.random_case(cells, 300 + q) for q in range(8)]
+    b = s.setup_case(cases[0])
+    net.reserve(len(cases))
+    for c in cases:
+        bc = s.setup_case(c)
+        # one right-hand side for the family: the same dipole on the same mesh
+        assert all(np.array_equal(x, y) for x, y in zip(bc, b))
+        net.add_sample(warmstart.features(c, bc), np.zeros((net.V, 8), np.float32))
+    s.setup_case(cases[0])
+    s.load_rhs(b)
+    losses, secs = s.train_residual(net, [c.cells["sigma"] for c in cases],
+                                    [c.omega for c in cases], steps=200, batch=2, lr=2e-3, seed=3)
+    print(f"[residual training] {secs:.1f}s loss {losses[:5].mean():.4f} -> "
+          f"{losses[-20:].mean():.4f}", flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < losses[:5].mean()
+    for c in cases[:3]:
+        bc = s.setup_case(c)
+        s.load_rhs(bc)
+        r0 = s.warm_start(net, warmstart.features(c, bc), jlsq=True)
+        assert r0 <= 1.0
+        r = s.solve_resident(1e-8, 20000, gpu.F_X0)
+        assert r.status == 0 and r.true_rel_res <= 1e-8
+    s.close()
+    net.close()
+
+
+def test_learned_warm_start_protocol_reports_its_gain():
+    """The learned warm start end to end on a small grid: solver-made
+    training data, two networks trained on the GPU (field MSE against the
+    solver's solutions; the residual of the network's own guess), held-out
+    models solved from x0 = 0 and from each guess.  Every warm-started solve
+    reaches true rel-res 1e-8 (the judge keeps the guess no worse than
+    x0 = 0), training lowers each loss, and the iteration ratios are
+    reported (not asserted: on this family they stay near 1)."""
+    from aphid_git_amd import warmstart
+
+    out = warmstart.protocol((16, 16, 16), ntrain=16, nhold=8, mse_steps=100, res_steps=100)
+    assert out["all_zero_converged"]
+    for name in ("field", "residual"):
+        o = out[name]
+        print(f"[protocol 16^3] {name}: iterations zero {out['iters_zero_mean']:.1f} warm "
+              f"{o['iters_warm_mean']:.1f} ratio {o['iters_ratio']:.3f} r0 median "
+              f"{o['r0_median']:.3e}; loss {o['loss_first']:.3f} -> {o['loss_last']:.3f}",
+              flush=True)
+        assert o["all_warm_converged"]
+        assert o["r0_median"] <= 1.0
+        assert o["loss_last"] < o["loss_first"]
