@@ -18,5 +18,5 @@ def test_default_parts_and_extra_parts():
     import bench
 
     assert bench.PARTS == ("solve", "config4", "shard", "batch", "unet", "warm", "cpu")
-    assert set(bench.EXTRA_PARTS) == {"batch3b", "warm5", "prec4"}
+    assert set(bench.EXTRA_PARTS) == {"batch3b", "warm5", "prec4", "prec4sweep"}
     assert bench.HBM_PEAK_GBS == 8000.0 and bench.BF16_PEAK_TFLOPS == 2500.0
