@@ -48,6 +48,8 @@ constexpr int kMaxPartBlocks = 1 << 16; // capacity of the partial-sum buffer
 // stencil launch shapes (compile-time tiles, runtime chunk)
 struct Tile {
   int tk, tj, chunk;  // chunk = x-planes per block (march); 0 = direct variant
+  int nt = 0;         // march: non-temporal stores of the output (the streams
+                      // tuner decides this inside the iteration's sequence)
 };
 // Launch shape of a streaming iteration kernel: elements unrolled per thread
 // and step, grid size (= number of reduction partials) and the
@@ -85,10 +87,13 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
   }
   const int chunk = t.chunk;
   dim3 grid((unsigned)ntk * ntj * ((op.nx + chunk - 1) / chunk));
-#define APHID_TILE(TK, TJ)                                                           \
-  if (t.tk == TK && t.tj == TJ) {                                                    \
-    hipLaunchKernelGGL((k_apply<R, TK, TJ>), grid, block, 0, s, op, in, out, chunk, ep); \
-    return hipGetLastError();                                                        \
+#define APHID_TILE(TK, TJ)                                                                 \
+  if (t.tk == TK && t.tj == TJ) {                                                          \
+    if (t.nt)                                                                              \
+      hipLaunchKernelGGL((k_apply<R, TK, TJ, true>), grid, block, 0, s, op, in, out, chunk, ep); \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_apply<R, TK, TJ, false>), grid, block, 0, s, op, in, out, chunk, ep); \
+    return hipGetLastError();                                                              \
   }
   APHID_TILE(16, 16)
   APHID_TILE(32, 8)
@@ -388,6 +393,7 @@ std::mutex g_tune_mu;
 std::map<TuneKey, Tile> g_tiles;
 struct StreamSet {
   Stream sh[SK_COUNT];
+  int snt = 0;  // the stencil's non-temporal output stores (Tile::nt)
 };
 std::map<TuneKey, StreamSet> g_streams;
 
@@ -611,10 +617,13 @@ int tune_streams(aphid_ctx* c) {
     const auto it = g_streams.find(tk0);
     if (it != g_streams.end()) {
       for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = it->second.sh[q];
+      c->tile[prec].nt = c->tile[prec].chunk > 0 ? it->second.snt : 0;
       c->stuned[prec] = true;
       return 0;
     }
   }
+  int rc = autotune(c, prec);  // the tile, before its store hint is set below
+  if (rc) return rc;
   char key[96];
   snprintf(key, sizeof key, "%d %d %d %d", c->nx, c->ny, c->nz, prec);
   const char* cache = getenv("APHID_TUNE_CACHE");
@@ -622,10 +631,12 @@ int tune_streams(aphid_ctx* c) {
     if (FILE* f = fopen(cache, "r")) {
       char line[200];
       while (fgets(line, sizeof line, f)) {
-        int a, b, cc, p, v[9];
-        if (sscanf(line, "S %d %d %d %d %d %d %d %d %d %d %d %d %d", &a, &b, &cc, &p, &v[0], &v[1],
-                   &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8]) != 13)
-          continue;
+        int a, b, cc, p, v[10];
+        v[9] = 0;  // lines without the stencil's store hint: plain stores
+        const int got = sscanf(line, "S %d %d %d %d %d %d %d %d %d %d %d %d %d %d", &a, &b, &cc,
+                               &p, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8],
+                               &v[9]);
+        if (got != 13 && got != 14) continue;
         char k2[96];
         snprintf(k2, sizeof k2, "%d %d %d %d", a, b, cc, p);
         if (strcmp(k2, key)) continue;
@@ -636,8 +647,9 @@ int tune_streams(aphid_ctx* c) {
           ok = ok && (sh[q].u == 2 || sh[q].u == 4) && sh[q].grid >= 1 &&
                sh[q].grid <= kMaxPartBlocks && (sh[q].nt == 0 || sh[q].nt == 1);
         }
-        if (ok) {
+        if (ok && (v[9] == 0 || v[9] == 1)) {
           for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = sh[q];
+          c->tile[prec].nt = c->tile[prec].chunk > 0 ? v[9] : 0;
           c->stuned[prec] = true;
         }
       }
@@ -645,12 +657,11 @@ int tune_streams(aphid_ctx* c) {
       if (c->stuned[prec]) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (int q = 0; q < SK_COUNT; ++q) g_streams[tk0].sh[q] = c->stream[prec][q];
+        g_streams[tk0].snt = c->tile[prec].nt;
         return 0;
       }
     }
   }
-  int rc = autotune(c, prec);
-  if (rc) return rc;
   rc = ensure_vectors(c, prec == 1);
   if (rc) return rc;
   const long long n4 = 4 * c->nn;
@@ -744,10 +755,31 @@ int tune_streams(aphid_ctx* c) {
     for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = cur[q];
   else
     for (int q = 0; q < SK_COUNT; ++q) c->stream[prec][q] = kStreamDefault[q];
+  // The march stencil's output with or without the non-temporal hint,
+  // timed in the same sequence: the hint keeps the stencil's re-read planes
+  // and halos in L2, but the next kernels read the output.  The hint does
+  // not change any value.  Kept when the sequence is 2% faster.
+  if (c->tile[prec].chunk > 0) {
+    Stream fin[SK_COUNT] = {c->stream[prec][0], c->stream[prec][1], c->stream[prec][2]};
+    float t0 = 0.f, t1 = 0.f;
+    c->tile[prec].nt = 0;
+    rc = seq_time(fin, &t0);
+    if (rc) return rc;
+    c->tile[prec].nt = 1;
+    rc = seq_time(fin, &t1);
+    if (rc) return rc;
+    c->tile[prec].nt = t1 < 0.98f * t0 ? 1 : 0;
+    if (const char* e = getenv("APHID_STENCIL_NT"))  // forced choice (measurements)
+      c->tile[prec].nt = e[0] == '1' ? 1 : 0;
+    if (tune_verbose())
+      fprintf(stderr, "[aphid] stencil stores prec %d: plain %.1f us, non-temporal %.1f us\n", prec,
+              1e3 * t0, 1e3 * t1);
+  }
   c->stuned[prec] = true;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (int q = 0; q < SK_COUNT; ++q) g_streams[tk0].sh[q] = c->stream[prec][q];
+    g_streams[tk0].snt = c->tile[prec].nt;
   }
   if (tune_verbose()) {
     fprintf(stderr, "[aphid] stream shapes prec %d (%.1f us, default %.1f):", prec, 1e3 * cur_ms,
@@ -763,7 +795,7 @@ int tune_streams(aphid_ctx* c) {
       for (int q = 0; q < SK_COUNT; ++q)
         fprintf(f, " %d %d %d", c->stream[prec][q].u, c->stream[prec][q].grid,
                 c->stream[prec][q].nt);
-      fprintf(f, "\n");
+      fprintf(f, " %d\n", c->tile[prec].nt);
       fclose(f);
     }
   return 0;

@@ -61,6 +61,20 @@ template <typename T>
 __device__ __forceinline__ void sto(T* __restrict__ base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+// the same store with the non-temporal hint (the nt bit of the global
+// store): the output streams to memory without taking L2 lines from the
+// inputs whose planes and halos the march re-reads
+template <bool NT, typename R>
+__device__ __forceinline__ void sto_h(cplx<R>* __restrict__ base, unsigned boff, cplx<R> v) {
+  if constexpr (NT) {
+    typedef R V2 __attribute__((ext_vector_type(2)));
+    V2* p = reinterpret_cast<V2*>(reinterpret_cast<char*>(base) + boff);
+    const V2 w = {v.re, v.im};
+    __builtin_nontemporal_store(w, p);
+  } else {
+    sto(base, boff, v);
+  }
+}
 
 // Loads through pointers that address global memory.  A pointer read from
 // a table in device memory (a batch's operator records) is generic to the
@@ -252,7 +266,7 @@ struct MarchAcc {
   }
 };
 
-template <typename R, int TK, int TJ>
+template <typename R, int TK, int TJ, bool NTS>
 __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* __restrict__ in,
                                                   cplx<R>* __restrict__ out, int chunk,
                                                   ApplyEpilogue ep) {
@@ -373,10 +387,10 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
       MarchAcc<R, TK, TJ> a{sm, pa, ca, na, ol, k2x_p, k2x_c, k2p_c, k2p_n};
       C rx, ry, rz, rp;
       rows4<R>(i, j, k, nx, ny, nz, X, Y, Z, a, rx, ry, rz, rp);
-      sto(out, ob, rx);
-      sto(out, nnb + ob, ry);
-      sto(out, 2 * nnb + ob, rz);
-      sto(out, 3 * nnb + ob, rp);
+      sto_h<NTS>(out, ob, rx);
+      sto_h<NTS>(out, nnb + ob, ry);
+      sto_h<NTS>(out, 2 * nnb + ob, rz);
+      sto_h<NTS>(out, 3 * nnb + ob, rp);
       if (ep.st) eps_acc<R>(ca, rx, ry, rz, rp, eacc_re, eacc_im);
     }
     __syncthreads();

@@ -221,6 +221,15 @@
           real(c_double), intent(out) :: seconds
         end function
 
+        integer(c_int) function aphid_batch_set_sweep_guess(batch,    &
+                                sys, nfrom, from)                      &
+                                bind(C, name='aphid_batch_set_sweep_guess')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: batch
+          integer(c_int), value :: sys, nfrom
+          integer(c_int), intent(in) :: from(*)
+        end function
+
         integer(c_int) function aphid_batch_get_solution(batch, sys,   &
                                 xx, xy, xz, xp)                        &
                                 bind(C, name='aphid_batch_get_solution')

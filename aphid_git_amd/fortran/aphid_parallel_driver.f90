@@ -4,7 +4,7 @@
 ! src/aphid_parallel.f90:40-193; formats in man/aphid.1:168-207) run as !
 ! one batched solve on one GPU instead of one MPI process per system.   !
 !                                                                      !
-!   aphid_gpu_parallel [-w dir]                                        !
+!   aphid_gpu_parallel [-w dir] [-s]                                   !
 !                                                                      !
 ! Reads dir/aphid_mpi.freq, dir/aphid_mpi.input, dir/aphid_mpi.source  !
 ! (CS jobs) and dir/aphid.mesh.  System p (0-based) is the reference's  !
@@ -14,7 +14,10 @@
 ! the digits of the system count, as the reference names its process   !
 ! directories): the generated aphid.input, qmr.iter (final iteration    !
 ! count and relative residuals; the batched solver keeps no history)    !
-! and aphid_solution.bin (raw A_x, A_y, A_z, Phi).                      !
+! and aphid_solution.bin (raw A_x, A_y, A_z, Phi).  With -s the batch   !
+! runs as a frequency sweep: per source, every second frequency in     !
+! ascending order starts from a fit of its two neighbours' solutions   !
+! (aphid_batch_set_sweep_guess).                                       !
 !----------------------------------------------------------------------!
       program aphid_gpu_parallel
       use iso_c_binding
@@ -41,8 +44,13 @@
       complex(c_double_complex), allocatable :: bz(:,:,:), bp(:,:,:)
       type(aphid_sys_report), allocatable :: reps(:)
       type(c_ptr) :: ctx, batch, pa, pb, pc, pd
+      logical :: sweep
+      integer, allocatable :: order(:)
+      integer :: g, nfrom
+      integer(c_int) :: from(2)
 
       wdir = '.'
+      sweep = .false.
       nargs = command_argument_count()
       ia = 1
       do while (ia <= nargs)
@@ -51,6 +59,7 @@
           call get_command_argument(ia + 1, wdir)
           ia = ia + 2
         else
+          if (arg(1:2) == '-s') sweep = .true.
           ia = ia + 1
         end if
       end do
@@ -204,6 +213,39 @@
         call check(rc, 'aphid_batch_add_system')
       end do
       rc = aphid_ctx_destroy(ctx)
+
+!     frequency sweep: per source, the frequencies in ascending order; each
+!     second one starts from its neighbours' solutions (same source)
+      if (sweep .and. nfreq >= 2) then
+        allocate(order(nfreq))
+        do f = 1, nfreq
+          order(f) = f
+        end do
+        do f = 2, nfreq
+          q = order(f)
+          g = f - 1
+          do while (g >= 1)
+            if (freqs(order(g)) <= freqs(q)) exit
+            order(g + 1) = order(g)
+            g = g - 1
+          end do
+          order(g + 1) = q
+        end do
+        do s = 1, nsys / nfreq
+          do f = 2, nfreq, 2
+            nfrom = 1
+            from(1) = int((s - 1) * nfreq + order(f - 1) - 1, c_int)
+            if (f < nfreq) then
+              nfrom = 2
+              from(2) = int((s - 1) * nfreq + order(f + 1) - 1, c_int)
+            end if
+            rc = aphid_batch_set_sweep_guess(batch,                       &
+                   int((s - 1) * nfreq + order(f) - 1, c_int),            &
+                   int(nfrom, c_int), from)
+            call check(rc, 'aphid_batch_set_sweep_guess')
+          end do
+        end do
+      end if
 
       allocate(reps(nsys))
       rc = aphid_batch_solve(batch, tol, int(max_it, c_int), 0_c_int, reps, secs)

@@ -30,7 +30,7 @@ from __future__ import annotations
 import dataclasses
 import os
 import time
-from typing import List, Optional, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -142,10 +142,26 @@ def tmp_dir_name(p: int, nprocs: int) -> str:
     return f"tmp-{p + 1:0{digits}d}"
 
 
+def sweep_guesses(freqs: Sequence[float], n_sources: int) -> Dict[int, List[int]]:
+    """The frequency-sweep schedule of a job: per source, the frequencies in
+    ascending order; every second one (system p = s*NFREQ + f) starts from
+    its neighbours' solutions.  Returns {second-wave system: source systems}."""
+    nf = len(freqs)
+    order = sorted(range(nf), key=lambda f: freqs[f])
+    out = {}
+    for s in range(n_sources):
+        for q in range(1, nf, 2):
+            nb = [order[q - 1]] + ([order[q + 1]] if q + 1 < nf else [])
+            out[s * nf + order[q]] = [s * nf + f for f in nb]
+    return out
+
+
 def run_job(d: str, device: int = 0, out_dir: Optional[str] = None, write: bool = True,
-            flags: int = 0):
-    """Solve every (source, frequency) of the job in d as one batched solve.
-    Returns (per-system reports, device seconds, wall seconds of the solve)."""
+            flags: int = 0, sweep: bool = False):
+    """Solve every (source, frequency) of the job in d as one batched solve
+    (sweep: every second frequency of each source, in ascending order,
+    starts from a fit of its neighbours' solutions).  Returns (per-system
+    reports, device seconds, wall seconds of the solve)."""
     job = read_job(d)
     out_dir = out_dir or d
     systems = job.systems()
@@ -159,6 +175,9 @@ def run_job(d: str, device: int = 0, out_dir: Optional[str] = None, write: bool 
         b, skipped = rhs.build_rhs(mode, dips)
         B.add_system(ops[p % nf], b)
     rhs.close()
+    if sweep:
+        for p, src in sweep_guesses(job.freqs, job.n_sources).items():
+            B.set_sweep_guess(p, src)
     t0 = time.perf_counter()
     reports, secs = B.solve(job.tol, job.max_it, flags=flags)
     wall = time.perf_counter() - t0

@@ -117,12 +117,13 @@ def config1(freq: float = 10.0, source: Optional[Tuple[int, int, int, int]] = No
                 "config1 32^3 uniform 50 m, air over 0.01 S/m half-space")
 
 
-def config2(seed: int = 2, freq: float = 1.0) -> Case:
+def config2(seed: int = 2, freq: float = 1.0, growth: float = 1.3) -> Case:
     """128x128x96 cells: 64x64x48 core at 25 m, 32 padding cells per side in
-    x,y and 32 below growing by 1.3, 16 air layers growing by 1.3 upwards;
-    0.01 S/m earth with 20 seeded blocks (2-20 core cells per axis,
-    log10 sigma ~ U[-4,0]); +X dipole at the surface centre; 1 Hz."""
-    h, g = 25.0, 1.3
+    x,y and 32 below growing by `growth` (1.3), 16 air layers growing by the
+    same factor upwards; 0.01 S/m earth with 20 seeded blocks (2-20 core
+    cells per axis, log10 sigma ~ U[-4,0]); +X dipole at the surface centre;
+    1 Hz."""
+    h, g = 25.0, growth
     xn = stretched_axis(64, h, 32, 32, g, -800.0)
     yn = stretched_axis(64, h, 32, 32, g, -800.0)
     zn = stretched_axis(48, h, 16, 32, g, 0.0)

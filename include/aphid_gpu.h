@@ -266,7 +266,8 @@ int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
  * condition x_a^T (b - K x0) = 0 on sys's operator (least squares on the
  * residual if that is worse than x0 = 0, and x0 = 0 if both are), and
  * solves the marked systems from those guesses (an x0 given for them is
- * replaced).  Source systems must be unmarked.  nfrom = 0 unmarks. */
+ * replaced).  Source systems must be unmarked.  nfrom = 0 unmarks.  A
+ * mixed-precision batch solve (APHID_F_MIXED) ignores the marks. */
 #define APHID_SWEEP_MAX_FROM 4
 int aphid_batch_set_sweep_guess(aphid_batch *batch, int sys, int nfrom,
                                 const int *from);

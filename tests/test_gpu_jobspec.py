@@ -49,17 +49,24 @@ def test_job_systems_match_single_solves(tmp_path, stype):
     s.close()
 
 
+@pytest.mark.parametrize("sweep", [False, True])
 @pytest.mark.parametrize("stype", ["CS", "MT"])
-def test_job_systems_match_the_oracle(tmp_path, oracle, stype):
+def test_job_systems_match_the_oracle(tmp_path, oracle, stype, sweep):
     """Every system of a job, solved by the batched HIP path, against the
     CPU oracle's complex128 QMR on the same (mode, frequency, source): the
     MT plane-wave modes 2 and 3 (src/aphid.f90:588-636) and CS dipole arrays
     (mode 5), 12^3 cells, rel-res 1e-10 on both sides.  Fields within 1e-7
-    (relative 2-norm over all unknowns)."""
+    (relative 2-norm over all unknowns).  With `sweep` every second
+    frequency of a source (in ascending order) starts from its neighbours'
+    solutions: its initial guess is better than x0 = 0 and its fields meet
+    the same bar."""
     arrays = [[(1, 7, 7, 4)], [(-3, 6, 7, 4), (2, 6, 8, 4)]] if stype == "CS" else []
-    freqs = [0.3, 30.0, 3000.0] if stype == "CS" else [1.0, 100.0, 1e4]
+    freqs = [30.0, 0.3, 3000.0, 3.0] if stype == "CS" else [1.0, 100.0, 1e4, 10.0]
     d, job = _job(tmp_path, stype, freqs, arrays)
-    reports, _, _ = jobspec.run_job(d)
+    reports, _, _ = jobspec.run_job(d, sweep=sweep)
+    second = jobspec.sweep_guesses(job.freqs, job.n_sources) if sweep else {}
+    for p in second:
+        assert reports[p]["r0_rel"] < 1.0, (p, reports[p])
     systems = job.systems()
     assert len(reports) == len(systems)
     for p, (mode, f, dips) in enumerate(systems):
@@ -72,8 +79,9 @@ def test_job_systems_match_the_oracle(tmp_path, oracle, stype):
         assert rep.status == 0
         xo = np.concatenate([a.ravel(order="F") for a in xo])
         err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
-        print(f"[job {stype}] system {p} mode {mode} f {f:g} Hz: iters {reports[p]['iters']} "
-              f"(oracle {rep.iters}), field err {err:.2e}", flush=True)
+        print(f"[job {stype} sweep={sweep}] system {p} mode {mode} f {f:g} Hz: iters "
+              f"{reports[p]['iters']} (oracle {rep.iters}), r0 {reports[p]['r0_rel']:.2e}, "
+              f"field err {err:.2e}", flush=True)
         assert err < 1e-7, (p, err)
 
 
@@ -81,11 +89,13 @@ PAR_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
                        "aphid_git_amd", "aphid_gpu_parallel")
 
 
+@pytest.mark.parametrize("sweep", [False, True])
 @pytest.mark.parametrize("stype", ["CS", "MT"])
-def test_fortran_job_driver_matches_run_job(tmp_path, stype):
+def test_fortran_job_driver_matches_run_job(tmp_path, stype, sweep):
     """aphid_gpu_parallel (Fortran over iso_c_binding) writes the same
     per-process directories as run_job: the same batched solve, so the same
-    iteration counts and the same solutions."""
+    iteration counts and the same solutions; with -s both run the same
+    frequency sweep."""
     import subprocess
     if not os.path.exists(PAR_EXE):
         pytest.fail("aphid_gpu_parallel is not built")
@@ -93,8 +103,9 @@ def test_fortran_job_driver_matches_run_job(tmp_path, stype):
     freqs = [0.5, 50.0] if stype == "CS" else [2.0, 200.0, 2e4]
     d, job = _job(tmp_path, stype, freqs, arrays)
     py_out = str(tmp_path / "py")
-    reports, _, _ = jobspec.run_job(d, out_dir=py_out)
-    r = subprocess.run([PAR_EXE, "-w", d], capture_output=True, text=True, timeout=300)
+    reports, _, _ = jobspec.run_job(d, out_dir=py_out, sweep=sweep)
+    r = subprocess.run([PAR_EXE, "-w", d] + (["-s"] if sweep else []), capture_output=True,
+                       text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     n = len(job.systems())
     assert f"Solved {n:6d} systems" in r.stdout
