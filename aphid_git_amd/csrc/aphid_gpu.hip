@@ -1008,11 +1008,11 @@ int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs)
 // host read of the device state at or before that iteration (0 before the
 // first), like the reference's t2 - t1 column, which it refreshes every 10
 // iterations (src/aphid_qmr.f90:376-381).
-void fill_hist_time(const aphid_ctx* c, double* t, int n) {
+void fill_hist_time(const std::vector<std::pair<int, double>>& marks, double* t, int n) {
   size_t m = 0;
   double last = 0.0;
   for (int q = 0; q < n; ++q) {
-    while (m < c->marks.size() && c->marks[m].first <= q + 1) last = c->marks[m++].second;
+    while (m < marks.size() && marks[m].first <= q + 1) last = marks[m++].second;
     t[q] = last;
   }
 }
@@ -1626,7 +1626,7 @@ int aphid_solve_resident(aphid_ctx* c, double tol, int max_it, unsigned flags,
     rep->hist_len = std::min(std::max(executed, 0), cap);
     if (cap && rep->hist_len > 0)
       CK(hipMemcpy(rep->hist, c->hist, sizeof(double) * rep->hist_len, hipMemcpyDeviceToHost));
-    if (rep->hist_time) fill_hist_time(c, rep->hist_time, rep->hist_len);
+    if (rep->hist_time) fill_hist_time(c->marks, rep->hist_time, rep->hist_len);
     rep->seconds_total =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -2229,6 +2229,8 @@ struct aphid_batch {
     // solutions of these systems (aphid_batch_set_sweep_guess)
     std::vector<int> from;
     double r0 = 1.0;  // ||b - K x0|| / ||b|| of its initial guess (NaN: not formed)
+    int hist_len = 0;                            // history entries of the last solve
+    std::vector<std::pair<int, double>> marks;   // (iterations done, seconds) of that solve
   };
   std::vector<Sys> sys;
   long long n4_tot = 0, op4_tot = 0;
@@ -2272,6 +2274,13 @@ struct aphid_batch {
   bool ntuned[2] = {false, false};
   QmrState* tst = nullptr;
   int* dskip = nullptr;  // [nsys] systems the batched launches leave alone
+  // residual history per system (aphid_batch_set_history): [nsys][hist_cap]
+  // (hist_cap: entries per system the solves keep; hist_alloc: per system
+  // in the allocation, which the captured iteration graph refers to)
+  double* hist = nullptr;
+  int hist_cap = 0, hist_alloc = 0;
+  const std::vector<int>* wave = nullptr;  // systems of the wave being solved (marks)
+  std::chrono::steady_clock::time_point t_solve;  // start of the batched QMR (marks)
 };
 
 namespace {
@@ -2314,13 +2323,15 @@ void batch_free_device(aphid_batch* B) {
   B->dop = nullptr; B->dbstart = nullptr; B->dtstart = nullptr; B->st = nullptr; B->part = nullptr;
   B->dbsys = nullptr; B->dtsys = nullptr;
   B->res = nullptr; B->tst = nullptr; B->dskip = nullptr;
+  if (B->hist) (void)hipFree(B->hist), B->hist = nullptr;
+  B->hist_alloc = 0;
   B->ntuned[0] = B->ntuned[1] = false;
   B->built = false;
 }
 
 BatchDev batch_dev(const aphid_batch* B) {
   return BatchDev{(int)B->sys.size(), B->doff, B->dn4, B->dop, B->dopoff, B->dbstart, B->dtstart,
-                  B->st, B->dbsys, B->dtsys};
+                  B->st, B->dbsys, B->dtsys, B->hist, B->hist_alloc};
 }
 
 // (re)build the flat device arrays after systems were added
@@ -2519,6 +2530,7 @@ int batch_tune_nt(aphid_batch* B) {
   const std::vector<QmrState> ts[2] = {std::vector<QmrState>(ns, t1), std::vector<QmrState>(ns, t2)};
   BatchDev D = batch_dev(B);
   D.st = B->tst;
+  D.hist = nullptr;
   cplx<V>*x, *v, *p, *t, *d, *s, *r;
   const cplx<V>* invd;
   const OpArgs<V>* ops;
@@ -2721,6 +2733,7 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
     x = B->F[0]; b = B->F[1]; v = B->F[2]; p = B->F[3]; t = B->F[4]; d = B->F[5]; s_ = B->F[6];
     r = B->F[7]; invd = B->invdf; ops = B->dops32;
   }
+  if (f64) B->t_solve = std::chrono::steady_clock::now();
   CK(hipMemsetAsync(B->st, 0, sizeof(QmrState) * ns, B->s));
   hipLaunchKernelGGL(kb_norm_b<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, b, B->part);
   if (itol)
@@ -2733,12 +2746,21 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
   hipLaunchKernelGGL(kb_fin_init_r, dim3(ns), dim3(kBlock), 0, B->s, D, B->part);
   hipLaunchKernelGGL(kb_init_v<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, r, invd, v, p, d,
                      s_, B->part);
-  hipLaunchKernelGGL(kb_fin_init_v, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, max_it, 0,
-                     maxits, skip);
+  hipLaunchKernelGGL(kb_fin_init_v, dim3(ns), dim3(kBlock), 0, B->s, D, B->part, max_it,
+                     f64 ? B->hist_cap : 0, maxits, skip);
   CK(hipGetLastError());
   int rc = batch_fetch_states(B);
   if (rc) return rc;
   if (batch_all_done(B)) return 0;
+  // wall-clock marks per system of this wave (complex128 solves with a
+  // history only): (iterations done, seconds since this batched QMR began)
+  auto mark = [&]() {
+    if (!f64 || !B->hist_cap || !B->wave) return;
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                   B->t_solve).count();
+    for (int q = 0; q < ns; ++q)
+      if ((*B->wave)[q]) B->sys[q].marks.push_back({B->hst[q].iter, t});
+  };
   const double bytes = 22.0 * (f64 ? 16.0 : 8.0) * (double)B->n4_tot;
   // about 30 ms of work per replay (the host reads every system's state
   // after each one); iterations past a system's stop return at once
@@ -2769,6 +2791,7 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
     }
     rc = batch_fetch_states(B);
     if (rc) return rc;
+    mark();
     if (batch_all_done(B)) {
       // systems that stopped after an odd iteration: x and r one step behind
       hipLaunchKernelGGL(kb_catch_up<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, x, d, r, s_);
@@ -2802,10 +2825,27 @@ int batch_solve_single(aphid_batch* B, int q, double tol, int max_it, bool use_x
   for (int i = 0; i < 9; ++i) c->V[i] = B->V[i] + sy.off;
   c->vborrowed = true;
   int rc = autotune(c, 0);
+  if (rc) return rc;
+  if (B->hist_cap > c->hist_cap) {
+    if (c->hist) CK(hipFree(c->hist));
+    CK(hipMalloc(&c->hist, sizeof(double) * B->hist_cap));
+    c->hist_cap = B->hist_cap;
+    for (auto& g : c->gexec)
+      if (g) (void)hipGraphExecDestroy(g), g = nullptr;  // graphs captured the old pointer
+  }
   double secs = 0.0;
-  if (!rc) rc = qmr_device<double>(c, tol, max_it, use_x0, &secs);
+  rc = qmr_device<double>(c, tol, max_it, use_x0, &secs);
   if (rc) return rc;
   *out = *c->hst;
+  if (B->hist_cap) {
+    auto& sy2 = B->sys[q];
+    const int executed = out->status == ST_MAXIT ? std::max(max_it, 1) : out->iter;
+    sy2.hist_len = std::min(std::max(executed, 0), B->hist_cap);
+    if (sy2.hist_len > 0)
+      CK(hipMemcpy(B->hist + (size_t)q * B->hist_alloc, c->hist, sizeof(double) * sy2.hist_len,
+                   hipMemcpyDeviceToDevice));
+    sy2.marks = c->marks;
+  }
   return 0;
 }
 
@@ -2835,12 +2875,20 @@ int batch_solve_wave(aphid_batch* B, const std::vector<int>& in_wave, double tol
   if (nbat) {
     if (!B->dskip) CK(hipMalloc(&B->dskip, sizeof(int) * ns));
     CK(hipMemcpyAsync(B->dskip, skip.data(), sizeof(int) * ns, hipMemcpyHostToDevice, B->s));
+    std::vector<int> marked(ns, 0);
+    for (int q = 0; q < ns; ++q) marked[q] = !skip[q];
+    B->wave = &marked;
     const int rc = batch_qmr<double>(B, tol, nullptr, max_it, use_x0, nullptr, B->dskip);
+    B->wave = nullptr;
     if (rc) return rc;
     for (int q = 0; q < ns; ++q)
       if (!skip[q]) {
         last[q] = B->hst[q];
         iters[q] = last[q].iter;
+        if (B->hist_cap) {
+          const int executed = last[q].status == ST_MAXIT ? std::max(max_it, 1) : last[q].iter;
+          B->sys[q].hist_len = std::min(std::max(executed, 0), B->hist_cap);
+        }
       }
   }
   return 0;
@@ -2953,6 +3001,20 @@ int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
   const bool use_x0 = (flags & APHID_F_X0) || any_x0;
   if (!use_x0) CK(hipMemsetAsync(B->V[0], 0, sizeof(c128) * B->n4_tot, B->s));
   B->solved = true;
+  if (B->hist_cap > B->hist_alloc) {
+    if (B->hist) {
+      CK(hipFree(B->hist));
+      B->hist = nullptr;
+    }
+    B->hist_alloc = 0;
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec), B->gexec = nullptr;  // captured the old pointer
+    CK(hipMalloc(&B->hist, sizeof(double) * (size_t)ns * B->hist_cap));
+    B->hist_alloc = B->hist_cap;
+  }
+  for (auto& sy : B->sys) {
+    sy.hist_len = 0;
+    sy.marks.clear();
+  }
   CK(hipStreamSynchronize(B->s));
   // wall clock: large systems run on their operator's own stream
   const auto t_start = std::chrono::steady_clock::now();
@@ -3088,6 +3150,29 @@ int aphid_batch_set_sweep_guess(aphid_batch* B, int sys, int nfrom, const int* f
     for (int f : B->sys[q].from)
       if (f == sys) return fail(APHID_E_ARG, "system is a source of another system's guess");
   B->sys[sys].from.assign(from, from + nfrom);
+  return 0;
+}
+
+int aphid_batch_set_history(aphid_batch* B, int cap) {
+  if (!B) return fail(APHID_E_ARG, "NULL batch");
+  if (cap < 0) return fail(APHID_E_ARG, "cap must be >= 0");
+  B->hist_cap = cap;
+  return 0;
+}
+
+int aphid_batch_get_history(aphid_batch* B, int sys, double* hist, double* time, int cap,
+                            int* len) {
+  if (!B || !len || (cap > 0 && !hist)) return fail(APHID_E_ARG, "NULL argument");
+  if (cap < 0) return fail(APHID_E_ARG, "cap must be >= 0");
+  if (sys < 0 || sys >= (int)B->sys.size()) return fail(APHID_E_ARG, "no such system");
+  const auto& sy = B->sys[sys];
+  const int n = B->hist ? std::min(std::min(cap, sy.hist_len), B->hist_alloc) : 0;
+  *len = n;
+  if (n <= 0) return 0;
+  CK(hipSetDevice(B->dev));
+  CK(hipMemcpy(hist, B->hist + (size_t)sys * B->hist_alloc, sizeof(double) * n,
+               hipMemcpyDeviceToHost));
+  if (time) fill_hist_time(sy.marks, time, n);
   return 0;
 }
 

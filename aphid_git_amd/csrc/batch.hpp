@@ -33,6 +33,10 @@ struct BatchDev {
   QmrState* st;             // [nsys]
   const int* bsys;          // [bstart[nsys]] system of each streaming work item
   const int* tsys;          // [tstart[nsys]] system of each stencil tile
+  // ||r|| per iteration, hist_stride entries per system (null: none); a
+  // system's state says how many of them its solve writes (hist_cap)
+  double* hist;
+  int hist_stride;
 };
 
 // streaming work item of this block: system, global element range [e0, e1)
@@ -495,7 +499,7 @@ __global__ void kb_fin_update(BatchDev B, const double* part) {
   if (st->done) return;
   double tot[3];
   b_fin_sum<3>(part, B.bstart[s], B.bstart[s + 1], tot);
-  if (threadIdx.x == 0) update_scalars(st, tot, nullptr);
+  if (threadIdx.x == 0) update_scalars(st, tot, B.hist ? B.hist + (size_t)s * B.hist_stride : nullptr);
 }
 
 // per-system ||b - t|| and ||b||

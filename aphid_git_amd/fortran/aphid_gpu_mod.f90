@@ -230,6 +230,23 @@
           integer(c_int), intent(in) :: from(*)
         end function
 
+        integer(c_int) function aphid_batch_set_history(batch, cap)    &
+                                bind(C, name='aphid_batch_set_history')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: batch
+          integer(c_int), value :: cap
+        end function
+
+        integer(c_int) function aphid_batch_get_history(batch, sys,    &
+                                hist, time, cap, len)                  &
+                                bind(C, name='aphid_batch_get_history')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: batch
+          integer(c_int), value :: sys, cap
+          real(c_double), intent(out) :: hist(*), time(*)
+          integer(c_int), intent(out) :: len
+        end function
+
         integer(c_int) function aphid_batch_get_solution(batch, sys,   &
                                 xx, xy, xz, xp)                        &
                                 bind(C, name='aphid_batch_get_solution')

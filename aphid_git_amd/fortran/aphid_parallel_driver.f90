@@ -12,9 +12,10 @@
 ! plane wave (serial modes 2, 3), for CS dipole array p div NFREQ       !
 ! (mode 5).  Each system's outputs go to dir/tmp-<p+1> (zero-padded to  !
 ! the digits of the system count, as the reference names its process   !
-! directories): the generated aphid.input, qmr.iter (final iteration    !
-! count and relative residuals; the batched solver keeps no history)    !
-! and aphid_solution.bin (raw A_x, A_y, A_z, Phi).  With -s the batch   !
+! directories): the generated aphid.input, qmr.iter (iteration,         !
+! residual norm, seconds, as each of the reference's processes writes   !
+! it, src/aphid_qmr.f90:381; at most 32M / NSYS lines per system) and   !
+! aphid_solution.bin (raw A_x, A_y, A_z, Phi).  With -s the batch       !
 ! runs as a frequency sweep: per source, every second frequency in     !
 ! ascending order starts from a fit of its two neighbours' solutions   !
 ! (aphid_batch_set_sweep_guess).                                       !
@@ -48,6 +49,8 @@
       integer, allocatable :: order(:)
       integer :: g, nfrom
       integer(c_int) :: from(2)
+      real(c_double), allocatable :: hist(:), htime(:)
+      integer(c_int) :: hcap, hlen
 
       wdir = '.'
       sweep = .false.
@@ -247,6 +250,12 @@
         end do
       end if
 
+!     residual histories for qmr.iter: one per system, 256 MB at most
+      hcap = int(max(1, min(max_it, 33554432 / nsys)), c_int)
+      allocate(hist(hcap), htime(hcap))
+      rc = aphid_batch_set_history(batch, hcap)
+      call check(rc, 'aphid_batch_set_history')
+
       allocate(reps(nsys))
       rc = aphid_batch_solve(batch, tol, int(max_it, c_int), 0_c_int, reps, secs)
       call check(rc, 'aphid_batch_solve')
@@ -283,10 +292,12 @@
           end do
           close(22)
         end if
+        rc = aphid_batch_get_history(batch, int(p, c_int), hist, htime, hcap, hlen)
+        call check(rc, 'aphid_batch_get_history')
         open(unit=23, file=trim(wdir)//'/'//trim(tname)//'/qmr.iter', status='replace')
-        write(23, '(i8,1x,es24.16,1x,es24.16,1x,i2)') reps(p+1)%iters,               &
-              reps(p+1)%rnorm / max(reps(p+1)%rhsnorm, tiny(1.0_dp)),                 &
-              reps(p+1)%true_rel_res, reps(p+1)%status
+        do q = 1, hlen
+          write(23, '(i8,1x,es24.16,1x,es24.16)') q, hist(q), htime(q)
+        end do
         close(23)
         rc = aphid_batch_get_solution(batch, int(p, c_int), bx, by, bz, bp)
         call check(rc, 'aphid_batch_get_solution')

@@ -455,6 +455,22 @@ class BatchSolver:
                                                  _p(arr) if len(arr) else None),
                "aphid_batch_set_sweep_guess")
 
+    def set_history(self, cap: int):
+        """Keep ||r|| of the first `cap` iterations of every system on
+        later complex128 solves (0: none)."""
+        _check(lib().aphid_batch_set_history(self._b, C.c_int(cap)), "aphid_batch_set_history")
+
+    def history(self, sys: int, cap: int):
+        """(||r|| per iteration, seconds at which each was read) of system
+        sys from the last solve, at most cap entries."""
+        h = np.zeros(max(cap, 0), np.float64)
+        t = np.zeros(max(cap, 0), np.float64)
+        n = C.c_int()
+        _check(lib().aphid_batch_get_history(self._b, sys, _p(h) if cap else None,
+                                             _p(t) if cap else None, C.c_int(cap), C.byref(n)),
+               "aphid_batch_get_history")
+        return h[:n.value].copy(), t[:n.value].copy()
+
 
 class UNet:
     """The warm-start network (aphid_unet_*): a 3-level 3-D U-Net on a cell
@@ -559,7 +575,8 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy",
            "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",
            "aphid_batch_add_system", "aphid_batch_solve", "aphid_batch_get_solution",
-           "aphid_batch_set_guess", "aphid_batch_set_sweep_guess",
+           "aphid_batch_set_guess", "aphid_batch_set_sweep_guess", "aphid_batch_set_history",
+           "aphid_batch_get_history",
            "aphid_unet_create", "aphid_unet_destroy", "aphid_unet_param_count",
            "aphid_unet_forward_flops", "aphid_unet_get_params", "aphid_unet_set_params",
            "aphid_unet_forward", "aphid_unet_loss_grad", "aphid_unet_activation",

@@ -19,9 +19,10 @@ process count) from a generated aphid.input.
 
 Here every (source, frequency) is one system of a BatchSolver: one operator
 per frequency (the mesh and model are shared), and one solve for all.
-run_job writes, per system, the generated aphid.input, qmr.iter (the
-residual history is not kept by the batched solver, so it holds the final
-iteration count and relative residuals) and aphid_solution.bin (A_x, A_y,
+run_job writes, per system, the generated aphid.input, qmr.iter
+(iteration, residual norm, seconds, one line per iteration as each of the
+reference's processes writes it, src/aphid_qmr.f90:381; at most
+HISTORY_ENTRIES // n_systems lines) and aphid_solution.bin (A_x, A_y,
 A_z, Phi in the reference's shapes, raw complex128 in Fortran order, as the
 serial driver aphid_gpu_serial writes).
 """
@@ -156,6 +157,10 @@ def sweep_guesses(freqs: Sequence[float], n_sources: int) -> Dict[int, List[int]
     return out
 
 
+# residual history kept over all systems of a job (256 MB of FP64)
+HISTORY_ENTRIES = 1 << 25
+
+
 def run_job(d: str, device: int = 0, out_dir: Optional[str] = None, write: bool = True,
             flags: int = 0, sweep: bool = False):
     """Solve every (source, frequency) of the job in d as one batched solve
@@ -178,6 +183,8 @@ def run_job(d: str, device: int = 0, out_dir: Optional[str] = None, write: bool 
     if sweep:
         for p, src in sweep_guesses(job.freqs, job.n_sources).items():
             B.set_sweep_guess(p, src)
+    hcap = max(1, min(job.max_it, HISTORY_ENTRIES // len(systems))) if write else 0
+    B.set_history(hcap)
     t0 = time.perf_counter()
     reports, secs = B.solve(job.tol, job.max_it, flags=flags)
     wall = time.perf_counter() - t0
@@ -196,10 +203,10 @@ def run_job(d: str, device: int = 0, out_dir: Optional[str] = None, write: bool 
                 with open(os.path.join(td, "aphid.source"), "w") as fo:
                     for (dr, i, j, k) in dips:
                         fo.write(f"{meshgen.DIR_NAMES[dr]} {i} {j} {k}\n")
-            r = reports[p]
+            hist, htime = B.history(p, hcap)
             with open(os.path.join(td, "qmr.iter"), "w") as fo:
-                fo.write(f"{r['iters']:8d} {r['rnorm'] / max(r['rhsnorm'], 1e-300):24.16e} "
-                         f"{r['true_rel_res']:24.16e} {r['status']:d}\n")
+                fo.writelines(f"{i + 1:8d} {h:24.16e} {t:24.16e}\n"
+                              for i, (h, t) in enumerate(zip(hist, htime)))
             x = B.solution(p)
             with open(os.path.join(td, "aphid_solution.bin"), "wb") as fo:
                 for a in x:

@@ -271,6 +271,22 @@ int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
 #define APHID_SWEEP_MAX_FROM 4
 int aphid_batch_set_sweep_guess(aphid_batch *batch, int sys, int nfrom,
                                 const int *from);
+/* Residual histories of the batched solve: every later complex128
+ * aphid_batch_solve keeps, per system, ||r|| of its first `cap` iterations
+ * and the wall-clock time at which each was read, the figures each process
+ * of the reference's farm writes into its own qmr.iter
+ * (src/aphid_qmr.f90:376-381 under src/aphid_parallel.f90:201).  cap = 0
+ * (the default) keeps none.  A mixed-precision solve (APHID_F_MIXED) keeps
+ * none (its inner iterations run in complex64 on a changing right-hand
+ * side). */
+int aphid_batch_set_history(aphid_batch *batch, int cap);
+/* History of system sys from the last solve: *len (<= cap) entries of
+ * ||r|| after iteration 1, 2, ... into hist and, when time is not NULL,
+ * the seconds since that system's QMR began (its wave's batched launches,
+ * or its own single-system solve) at the latest host read of its state at
+ * or before that iteration (0 before the first read). */
+int aphid_batch_get_history(aphid_batch *batch, int sys, double *hist,
+                            double *time, int cap, int *len);
 
 /* Post-processing of the resident solution (gen_fields, src/aphid.f90:
  * 2173-2252): B = curl A on the primary cell faces -- Bx (nx,ny-1,nz-1),

@@ -276,6 +276,68 @@ def test_large_systems_take_the_single_path_in_place(gpu_mod, monkeypatch):
     single.close()
 
 
+def test_residual_histories_of_batched_and_single_path_systems(gpu_mod, monkeypatch):
+    """aphid_batch_set_history / get_history: each system's ||r|| per
+    iteration (the figures each of the reference's farm processes writes
+    into its qmr.iter, src/aphid_qmr.f90:381).  A large system on the
+    single-system path keeps Solver.qmr_solve's history exactly; batched
+    systems keep theirs (first 20 entries to 1e-6 of the single solve's);
+    one entry per iteration, times non-decreasing.  A solve stopped by
+    max_it keeps min(max_it, cap) entries; cap 0 and mixed-precision solves
+    keep none."""
+    base = meshgen.config1()
+    monkeypatch.setenv("APHID_BATCH_SEQ_MIN", "100000")
+    small = _uneven_cases()[:2]
+    single = gpu_mod.Solver()
+    with gpu_mod.BatchSolver() as B:
+        plan = []
+        ob = B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * 3.0, base.cells)
+        single.set_mesh(base.xn, base.yn, base.zn)
+        b, _ = single.build_rhs(5, [(1, 17, 17, 9)])
+        plan.append((base, 2 * np.pi * 3.0, "R", base.cells, b, True))
+        ops_small = [B.add_case_operator(c) for c, _ in small]
+        B.add_system(ob, b)
+        for o, (c, ss) in zip(ops_small, small):
+            single.set_mesh(c.xn, c.yn, c.zn)
+            bs, _ = single.build_rhs(5, [ss[0]])
+            B.add_system(o, bs)
+            plan.append((c, c.omega, c.model, c.cells, bs, False))
+        cap = 6000
+        B.set_history(cap)
+        reps, _ = B.solve(1e-9, 5000)
+        for q, (c, om, model, cells, bq, big) in enumerate(plan):
+            r = reps[q]
+            assert r["status"] == 0, (q, r)
+            h, t = B.history(q, cap)
+            print(f"[batch history] system {q} big={big}: {len(h)} entries, iters {r['iters']}, "
+                  f"last t {t[-1]:.4f}s", flush=True)
+            assert len(h) == r["iters"] and len(t) == len(h)
+            assert np.all(np.diff(t) >= 0) and t[0] >= 0
+            assert h[-1] < 1e-9 * r["rhsnorm"] <= h[-2]
+            single.set_mesh(c.xn, c.yn, c.zn)
+            single.set_cells(model, om, cells)
+            rs = single.qmr_solve(bq, 1e-9, 5000, hist_cap=cap)
+            if big:
+                assert rs.iters == r["iters"]
+                assert np.allclose(h, rs.hist, rtol=1e-12, atol=0)
+            else:
+                m = min(20, len(h), len(rs.hist))
+                assert np.allclose(h[:m], rs.hist[:m], rtol=1e-6, atol=0)
+        B.set_history(5)
+        reps, _ = B.solve(1e-9, 7)
+        for q in range(len(plan)):
+            assert reps[q]["status"] == 1
+            h, t = B.history(q, 100)
+            assert len(h) == 5 and np.all(h > 0)
+        B.set_history(0)
+        B.solve(1e-9, 20)
+        assert all(len(B.history(q, 100)[0]) == 0 for q in range(len(plan)))
+        B.set_history(100)
+        B.solve(1e-9, 5000, flags=gpu_mod.F_MIXED)
+        assert all(len(B.history(q, 100)[0]) == 0 for q in range(len(plan)))
+    single.close()
+
+
 def _sweep_family(nf=9, seed=61):
     """One 16^3-cell random-block model at nf log-spaced frequencies
     (1e-1..1e3 Hz) x two dipoles: systems ordered frequency-major."""

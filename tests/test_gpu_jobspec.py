@@ -59,7 +59,10 @@ def test_job_systems_match_the_oracle(tmp_path, oracle, stype, sweep):
     (relative 2-norm over all unknowns).  With `sweep` every second
     frequency of a source (in ascending order) starts from its neighbours'
     solutions: its initial guess is better than x0 = 0 and its fields meet
-    the same bar."""
+    the same bar.  Each tmp-N/qmr.iter holds one line per iteration
+    (src/aphid_qmr.f90:381); from x0 = 0 its first 20 residual norms equal
+    the oracle's history to 1e-6 and its length is within 10% of the
+    oracle's iteration count."""
     arrays = [[(1, 7, 7, 4)], [(-3, 6, 7, 4), (2, 6, 8, 4)]] if stype == "CS" else []
     freqs = [30.0, 0.3, 3000.0, 3.0] if stype == "CS" else [1.0, 100.0, 1e4, 10.0]
     d, job = _job(tmp_path, stype, freqs, arrays)
@@ -75,8 +78,17 @@ def test_job_systems_match_the_oracle(tmp_path, oracle, stype, sweep):
         x = np.fromfile(os.path.join(td, "aphid_solution.bin"), np.complex128)
         case = meshgen.Case(job.xn, job.yn, job.zn, job.model, job.cells, f, mode, list(dips),
                             job.tol, job.max_it)
-        xo, rep, _ = oracle.System(case).solve(tol=job.tol, max_it=job.max_it)
+        xo, rep, ohist = oracle.System(case).solve(tol=job.tol, max_it=job.max_it)
         assert rep.status == 0
+        it = np.loadtxt(os.path.join(td, "qmr.iter"), ndmin=2)
+        assert it.shape == (reports[p]["iters"], 3)
+        assert np.array_equal(it[:, 0], np.arange(1, len(it) + 1))
+        assert np.all(np.diff(it[:, 2]) >= 0)
+        if p not in second:
+            m = min(20, len(ohist), len(it))
+            rel = np.abs(it[:m, 1] - ohist[:m]) / ohist[:m]
+            assert rel.max() < 1e-6, (p, rel.max())
+            assert abs(len(it) - rep.iters) <= max(3, 0.1 * rep.iters), (len(it), rep.iters)
         xo = np.concatenate([a.ravel(order="F") for a in xo])
         err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
         print(f"[job {stype} sweep={sweep}] system {p} mode {mode} f {f:g} Hz: iters "
@@ -116,9 +128,13 @@ def test_fortran_job_driver_matches_run_job(tmp_path, stype, sweep):
         xp = np.fromfile(os.path.join(pd, "aphid_solution.bin"), np.complex128)
         assert xf.shape == xp.shape
         assert np.linalg.norm(xf - xp) <= 1e-12 * np.linalg.norm(xp), p
-        it = open(os.path.join(fd, "qmr.iter")).read().split()
-        assert int(it[0]) == reports[p]["iters"] and int(it[3]) == 0
-        assert float(it[2]) <= 1e-10
+        itf = np.loadtxt(os.path.join(fd, "qmr.iter"), ndmin=2)
+        itp = np.loadtxt(os.path.join(pd, "qmr.iter"), ndmin=2)
+        assert itf.shape == itp.shape == (reports[p]["iters"], 3)
+        assert np.array_equal(itf[:, 0], np.arange(1, len(itf) + 1))
+        assert np.allclose(itf[:, 1], itp[:, 1], rtol=1e-9, atol=0)
+        assert itf[-1, 1] < job.tol * reports[p]["rhsnorm"]
+        assert np.all(np.diff(itf[:, 2]) >= 0) and np.all(itf[:, 2] >= 0)
         inp = open(os.path.join(fd, "aphid.input")).read().splitlines()
         assert int(inp[3].split()[0]) == mode
         assert float(inp[6].split()[0]) == pytest.approx(f, rel=1e-6)
