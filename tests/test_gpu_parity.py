@@ -242,13 +242,13 @@ def test_config2_fields_match_the_oracle(gpu):
     oracle's 4-thread complex128 solve of the same system (which follows
     the reference's arithmetic order; tests/golden/config2_1Hz.npz holds
     its iteration count, residual history and a seeded 1% sample of its
-    solution).  Both solves run the same Krylov process from x0 = 0, but
-    their reductions round differently, and on this system's long plateau
-    (residual near 2e-3 from ~500 to ~15,000 iterations) the last bits
-    steer the iterates apart; the two rel-res-1e-8 solutions then differ
-    by what that tolerance leaves of the field error.  The test states the
-    measured gap: sampled fields within 1e-6 (relative 2-norm), component
-    norms within 1e-6, iteration counts within 10%."""
+    solution).  Both solves run the same Krylov process from x0 = 0; their
+    reductions round differently, so on this system's long plateau
+    (residual near 2e-3 from ~500 to ~15,000 iterations) the iteration
+    counts drift apart (46,315 against 47,035), but both converged
+    solutions agree far inside the north-star bar: sampled fields within
+    1e-7 (relative 2-norm; measured 1.5e-10), component norms within 1e-7
+    (measured 4.9e-10), iteration counts within 10%."""
     case = meshgen.config2()
     z = _config2_golden()
     b = gpu.setup_case(case)
@@ -263,7 +263,7 @@ def test_config2_fields_match_the_oracle(gpu):
           f"{r.true_rel_res:.3e} (oracle {float(z['oracle_true_rel_res']):.3e}); sampled field "
           f"difference {err:.3e}, component norms {nerr:.3e}", flush=True)
     assert abs(r.iters - int(z["oracle_iters"])) <= 0.1 * int(z["oracle_iters"])
-    assert err < 1e-6 and nerr < 1e-6
+    assert err < 1e-7 and nerr < 1e-7
 
 
 # --------------------------------------------------------------------------
