@@ -552,13 +552,15 @@ hipError_t launch_pupdate(const Stream& sh, const cplx<V>* v, const cplx<V>* inv
   return hipErrorInvalidValue;
 }
 
+// `mid` (optional): an event recorded between the kernel and its finalize
 template <typename V>
 hipError_t launch_vtnorm(const Stream& sh, const cplx<V>* t, const cplx<V>* v, long long n,
-                         QmrState* st, double* part, hipStream_t s) {
+                         QmrState* st, double* part, hipStream_t s, hipEvent_t mid = nullptr) {
 #define X(U, NT)                                                                              \
   if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
     hipLaunchKernelGGL((k_vtnorm<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, t, v, n, st,    \
                        part);                                                                 \
+    if (mid) (void)hipEventRecord(mid, s);                                                    \
     hipLaunchKernelGGL(k_fin_vtnorm, dim3(1), dim3(kBlock), 0, s, part, sh.grid, st);          \
     return hipGetLastError();                                                                 \
   }
@@ -569,11 +571,13 @@ hipError_t launch_vtnorm(const Stream& sh, const cplx<V>* t, const cplx<V>* v, l
 
 template <typename V>
 hipError_t launch_update(const Stream& sh, const Vecs<V>& w, const cplx<V>* invd, long long n,
-                         QmrState* st, double* part, double* hist, hipStream_t s) {
+                         QmrState* st, double* part, double* hist, hipStream_t s,
+                         hipEvent_t mid = nullptr) {
 #define X(U, NT)                                                                              \
   if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
     hipLaunchKernelGGL((k_update<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, w.p, w.d, w.x,  \
                        w.t, w.s, w.r, w.v, invd, n, st, part);                                \
+    if (mid) (void)hipEventRecord(mid, s);                                                    \
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, s, part, sh.grid, st, hist);    \
     return hipGetLastError();                                                                 \
   }
@@ -2127,44 +2131,61 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   const Vecs<double> w = vecs<double>(c);
   const long long n4 = 4 * c->nn;
   const Stream* sh = c->stream[0];
+  // ev (optional): events recorded between a kernel and its finalize
   auto K0 = [&]() {
     return launch_pupdate<double>(sh[SK_PUPDATE], w.v, c->invd, w.p, n4, c->st, c->s);
   };
-  auto K1 = [&]() {
+  auto K1 = [&](hipEvent_t mid) {
     ApplyEpilogue ep{c->st, c->part};
     const hipError_t e = launch_apply<double>(c->tile[0], c->op64, w.p, w.t, ep, c->s);
+    if (mid) (void)hipEventRecord(mid, c->s);
     hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
                        (int)apply_blocks(c->tile[0], c->nx, c->ny, c->nz), c->st);
     return e;
   };
-  auto K2 = [&]() {
-    return launch_vtnorm<double>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s);
+  auto K2 = [&](hipEvent_t mid) {
+    return launch_vtnorm<double>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s, mid);
   };
-  auto K3 = [&]() {
-    return launch_update<double>(sh[SK_UPDATE], w, c->invd, n4, c->st, c->part, nullptr, c->s);
+  auto K3 = [&](hipEvent_t mid) {
+    return launch_update<double>(sh[SK_UPDATE], w, c->invd, n4, c->st, c->part, nullptr, c->s,
+                                 mid);
   };
+  // each kernel's own duration inside the iteration's back-to-back
+  // sequence (the events between kernels mark one's end and the next's
+  // start), without the one-block finalize kernels: ev[0] | p-update |
+  // ev[1] | stencil | ev[2] | fin | ev[3] | vtnorm | ev[4] | fin | ev[5] |
+  // update | ev[6] | fin | ev[7]
+  hipEvent_t ev[8] = {};
+  for (auto& e : ev) CK(hipEventCreate(&e));
   float ms[5] = {0, 0, 0, 0, 0};
-  for (int rep = 0; rep < reps + 1; ++rep) {  // first sweep warms up
-    for (int q = 0; q < 4; ++q) {
-      CK(hipEventRecord(c->ev0, c->s));
-      if (q == 0) CK(K0());
-      if (q == 1) CK(K1());
-      if (q == 2) CK(K2());
-      if (q == 3) CK(K3());
-      CK(hipEventRecord(c->ev1, c->s));
-      CK(hipEventSynchronize(c->ev1));
+  hipError_t te = hipSuccess;
+  for (int rep = 0; rep < reps + 1 && te == hipSuccess; ++rep) {  // first sweep warms up
+    (void)hipEventRecord(ev[0], c->s);
+    te = K0();
+    (void)hipEventRecord(ev[1], c->s);
+    if (te == hipSuccess) te = K1(ev[2]);
+    (void)hipEventRecord(ev[3], c->s);
+    if (te == hipSuccess) te = K2(ev[4]);
+    (void)hipEventRecord(ev[5], c->s);
+    if (te == hipSuccess) te = K3(ev[6]);
+    (void)hipEventRecord(ev[7], c->s);
+    if (te == hipSuccess) te = hipEventSynchronize(ev[7]);
+    const int from[4] = {0, 1, 3, 5}, to[4] = {1, 2, 4, 6};
+    for (int q = 0; q < 4 && te == hipSuccess && rep; ++q) {
       float t = 0.f;
-      CK(hipEventElapsedTime(&t, c->ev0, c->ev1));
-      if (rep) ms[q] += t;
+      te = hipEventElapsedTime(&t, ev[from[q]], ev[to[q]]);
+      ms[q] += t;
     }
   }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (te != hipSuccess) return fail(APHID_E_HIP, std::string("timing: ") + hipGetErrorString(te));
   CK(hipGetLastError());
   CK(hipEventRecord(c->ev0, c->s));
   for (int rep = 0; rep < reps; ++rep) {
     CK(K0());
-    CK(K1());
-    CK(K2());
-    CK(K3());
+    CK(K1(nullptr));
+    CK(K2(nullptr));
+    CK(K3(nullptr));
   }
   CK(hipEventRecord(c->ev1, c->s));
   CK(hipEventSynchronize(c->ev1));

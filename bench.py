@@ -574,15 +574,17 @@ def main():
             # PMC traffic of the same kernel and tile only
             if list(tj.get("tile", [])) == list(tile):
                 traffic = tj.get("stencil_bytes_per_launch")
-        # the roofline entry times the stencil as the iteration launches it
-        # (with the eps = p^T K p epilogue, inside the iteration's own kernel
-        # sequence); the plain apply alone is reported beside it
+        # the roofline entry times the stencil kernel as the iteration
+        # launches it (with the eps = p^T K p epilogue, inside the iteration's
+        # back-to-back kernel sequence, without the one-block finalize kernel
+        # that follows it: the duration rocprofv3 reports for k_apply); the
+        # plain apply alone is reported beside it
         t_st = kt[1]
         out.update({
             "value": converged / elapsed,
             "ms_per_step": 1e3 * elapsed / args.steps,
             "roofline": {"bound": "hbm", "kernel": "stencil (k_apply with the eps epilogue, "
-                                                   "in the QMR iteration)",
+                                                   "in the QMR iteration; finalize excluded)",
                          "achieved": B_mv / t_st / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": B_mv / t_st / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
                          "traffic_source": "profiles/traffic_config2.json: rocprofv3 --pmc "

@@ -195,9 +195,11 @@ int aphid_set_tile(aphid_ctx *ctx, int precision, int tk, int tj, int chunk);
 int aphid_time_apply(aphid_ctx *ctx, int precision, int reps, double *seconds);
 
 /* Mean device time (seconds) of each kernel of one QMR iteration, timed on
- * the context's vectors after a solve: secs[0] p-update, [1] stencil with
- * the eps reduction, [2] ||t - beta v||, [3] fused update, [4] the whole
- * iteration.  Overwrites the resident iterate. */
+ * the context's vectors after a solve, each inside the iteration's
+ * back-to-back kernel sequence: secs[0] p-update, [1] stencil with the eps
+ * epilogue, [2] ||t - beta v||, [3] fused update (their one-block finalize
+ * kernels excluded), [4] the whole iteration (finalize kernels included).
+ * Overwrites the resident iterate. */
 int aphid_time_iteration(aphid_ctx *ctx, int precision, int reps, double *secs);
 
 /* Mean device time of a plain device-to-device copy kernel over `bytes`
