@@ -2114,6 +2114,52 @@ int aphid_time_apply(aphid_ctx* c, int precision, int reps, double* seconds) {
   return 0;
 }
 
+int aphid_time_traffic(aphid_ctx* c, int reps, double* seconds) {
+  if (!c || !seconds || reps < 1) return fail(APHID_E_ARG, "bad argument");
+  if (!c->have_k2) return fail(APHID_E_STATE, "operator not set up");
+  CK(hipSetDevice(c->dev));
+  int rc = ensure_vectors(c, false);
+  if (rc) return rc;
+  rc = autotune(c, 0);
+  if (rc) return rc;
+  const Tile& t = c->tile[0];
+  if (t.chunk <= 0) return fail(APHID_E_STATE, "the tuned stencil is the direct variant");
+  const OpArgs<double>& op = c->op64;
+  const int ntk = (op.nz + t.tk - 1) / t.tk, ntj = (op.ny + t.tj - 1) / t.tj;
+  const dim3 grid((unsigned)ntk * ntj * ((op.nx + t.chunk - 1) / t.chunk)), block(t.tk * t.tj);
+  auto launch = [&]() -> hipError_t {
+#define APHID_TILE(TK, TJ)                                                                  \
+    if (t.tk == TK && t.tj == TJ) {                                                         \
+      if (t.nt)                                                                             \
+        hipLaunchKernelGGL((k_apply_traffic<double, TK, TJ, true>), grid, block, 0, c->s,   \
+                           op, c->V[3], c->V[8], t.chunk);                                  \
+      else                                                                                  \
+        hipLaunchKernelGGL((k_apply_traffic<double, TK, TJ, false>), grid, block, 0, c->s,  \
+                           op, c->V[3], c->V[8], t.chunk);                                  \
+      return hipGetLastError();                                                             \
+    }
+    APHID_TILE(16, 16)
+    APHID_TILE(32, 8)
+    APHID_TILE(64, 4)
+    APHID_TILE(16, 8)
+    APHID_TILE(32, 4)
+    APHID_TILE(32, 16)
+    APHID_TILE(64, 8)
+    APHID_TILE(16, 32)
+#undef APHID_TILE
+    return hipErrorInvalidValue;
+  };
+  CK(launch());  // warm-up
+  CK(hipEventRecord(c->ev0, c->s));
+  for (int r = 0; r < reps; ++r) CK(launch());
+  CK(hipEventRecord(c->ev1, c->s));
+  CK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *seconds = 1e-3 * ms / reps;
+  return 0;
+}
+
 int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   if (!c || !secs || reps < 1 || precision != 0) return fail(APHID_E_ARG, "bad argument");
   if (!c->have_k2 || !c->V[0]) return fail(APHID_E_STATE, "solve once first");

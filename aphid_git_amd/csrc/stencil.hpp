@@ -412,6 +412,91 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
 }
 
 // ---------------------------------------------------------------------------
+// The x-march's memory traffic alone, for measurement: the same tiles, grid,
+// XCD order and chunks, the same loads (own columns of the four fields and
+// four k^2 grids, the halo cells' fields and one k^2 grid) and the same four
+// stores per owned node (with the same non-temporal hint), but no LDS
+// staging, barriers or row arithmetic.  It holds as much LDS as
+// k_apply<R,TK,TJ,NTS>, so the same number of blocks share a CU.  Its time is the floor the march's access pattern sets; the loaded
+// values are folded into the (meaningless) outputs so that no load is
+// dropped.
+template <typename R, int TK, int TJ, bool NTS>
+__global__ __launch_bounds__(TK* TJ) void k_apply_traffic(OpArgs<R> op,
+                                                          const cplx<R>* __restrict__ in,
+                                                          cplx<R>* __restrict__ out, int chunk) {
+  using C = cplx<R>;
+  constexpr int LK = TK + 2, LJ = TJ + 2, LS = LK * LJ;
+  constexpr int NH = 2 * TK + 2 * TJ;
+  constexpr unsigned CS = sizeof(C);
+  // k_apply's LDS: 7 planes and the 1-D tables
+  __shared__ C hold[7 * LS + (8 * (TJ + TK) * sizeof(R) + CS - 1) / CS];
+  const int nx = op.nx, ny = op.ny, nz = op.nz;
+  const unsigned pl = (unsigned)ny * nz;
+  const unsigned nnb = (unsigned)op.nn * CS, plb = pl * CS;
+  const int ntk = (nz + TK - 1) / TK, ntj = (ny + TJ - 1) / TJ;
+  const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+  const int bk = (int)(t % ntk), bj = (int)((t / ntk) % ntj);
+  const int bi = (int)(t / ((unsigned)ntk * ntj));
+  const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
+  const int k = bk * TK + tk, j = bj * TJ + tj;
+  const int i0 = bi * chunk, i1 = min(nx, i0 + chunk);
+  const bool own = (j < ny) && (k < nz);
+  const unsigned colb = own ? ((unsigned)j * nz + k) * CS : 0u;
+  int hj = 0, hk = 0;
+  const bool hv = threadIdx.x < NH;
+  if (hv) {
+    const int h = threadIdx.x;
+    if (h < TK) { hj = -1; hk = h; }
+    else if (h < 2 * TK) { hj = TJ; hk = h - TK; }
+    else if (h < 2 * TK + TJ) { hj = h - 2 * TK; hk = -1; }
+    else { hj = h - 2 * TK - TJ; hk = TK; }
+  }
+  const int gj = bj * TJ + hj, gk = bk * TK + hk;
+  const bool hin = hv && gj >= 0 && gj < ny && gk >= 0 && gk < nz;
+  const int hc = hj == -1 ? 1 : (hk == -1 ? 2 : 3);
+  const unsigned hcolb = hin ? ((unsigned)gj * nz + gk) * CS : 0u;
+  const C* __restrict__ K2 = op.k2;
+  auto ldc = [&](const C* __restrict__ base, int c, int i) -> C {
+    return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb)
+                                     : C{R(0), R(0)};
+  };
+  hold[threadIdx.x] = C{R(0), R(0)};
+  C acc = ldc(K2, 0, i0 - 1);
+  C ca[4], na[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    acc += ldc(in, c, i0 - 1);
+    ca[c] = ldc(in, c, i0);
+  }
+  C kx = ldc(K2, 0, i0), kp = ldc(K2, 3, i0);
+  for (int i = i0; i < i1; ++i) {
+    const C ky = ldc(K2, 1, i), kz = ldc(K2, 2, i);
+    C h = {R(0), R(0)};
+    if (hin) {
+      const unsigned g = (unsigned)i * plb + hcolb;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) h += ldo(in, c * nnb + g);
+      h += ldo(K2, hc * nnb + g);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
+    const C kxn = ldc(K2, 0, i + 1), kpn = ldc(K2, 3, i + 1);
+    if (own) {
+      const unsigned ob = (unsigned)i * plb + colb;
+      const C s = acc + h + kx + kp + ky + kz;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sto_h<NTS>(out, c * nnb + ob, ca[c] + s);
+    }
+    acc = C{acc.re * R(0.5), h.im};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ca[c] = na[c];
+    kx = kxn;
+    kp = kpn;
+  }
+  if (acc.re == R(1.5e300)) out[0] = hold[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
 // Variant 1: direct.  One node per thread, every neighbour read through the
 // L1/L2 caches; blocks are TK x TJ tiles of one x-plane.  The linear block id
 // is remapped so that the blocks one XCD receives (ids congruent mod 8) walk

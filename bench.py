@@ -547,6 +547,7 @@ def main():
 
         # kernel timing (HIP events on the library's stream)
         t_apply = s.time_apply(0, 20)
+        t_traffic = s.time_traffic(20)
         kt = s.time_iteration(0, 10)
         tile = s.get_tile(0)
         s_bytes = 16
@@ -593,6 +594,18 @@ def main():
                          "algorithmic_bytes_per_launch": B_mv, "seconds_per_launch": t_st,
                          "plain_apply_seconds": t_apply,
                          "plain_apply_frac": B_mv / t_apply / 1e9 / HBM_PEAK_GBS,
+                         # the same tile's loads and stores without arithmetic,
+                         # repeated like the plain apply: the floor its access
+                         # pattern sets
+                         "pattern_ceiling": {
+                             "seconds": t_traffic,
+                             "GBps": B_mv / t_traffic / 1e9,
+                             "frac_of_peak": B_mv / t_traffic / 1e9 / HBM_PEAK_GBS,
+                             "plain_apply_over_ceiling": t_traffic / t_apply,
+                             "definition": "aphid_time_traffic: the tuned march tile's loads "
+                                           "(own columns of 4 fields and 4 k^2 grids, halo "
+                                           "cells) and 4 stores per node, no LDS, barriers "
+                                           "or arithmetic, same occupancy"},
                          "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]}},
             "roofline_dominant_kernel": {"kernel": dom, **kernels[dom], "peak": HBM_PEAK_GBS},
             "kernels": kernels,

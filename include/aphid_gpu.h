@@ -194,6 +194,15 @@ int aphid_set_tile(aphid_ctx *ctx, int precision, int tk, int tj, int chunk);
  * vectors and return the mean kernel time in seconds. */
 int aphid_time_apply(aphid_ctx *ctx, int precision, int reps, double *seconds);
 
+/* Mean seconds per launch of the stencil's memory traffic alone: the tuned
+ * complex128 march tile's loads (own columns of the four fields and four
+ * k^2 grids, halo cells) and stores (with its store hint) with no
+ * arithmetic, LDS staging or barriers, at the same occupancy, repeated on the context's vectors (the
+ * spare vector receives meaningless values).  The floor the stencil's
+ * access pattern sets under the HBM spec.  No counterpart in the reference;
+ * for benchmarks. */
+int aphid_time_traffic(aphid_ctx *ctx, int reps, double *seconds);
+
 /* Mean device time (seconds) of each kernel of one QMR iteration, timed on
  * the context's vectors after a solve, each inside the iteration's
  * back-to-back kernel sequence: secs[0] p-update, [1] stencil with the eps

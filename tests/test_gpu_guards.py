@@ -83,3 +83,38 @@ def test_bad_tune_cache_lines_are_ignored(tmp_path, oracle):
             del os.environ["APHID_TUNE_CACHE"]
         else:
             os.environ["APHID_TUNE_CACHE"] = old
+
+
+def test_traffic_hook_times_and_leaves_the_solve_alone():
+    """aphid_time_traffic (the tuned stencil tile's loads and stores without
+    arithmetic) returns a positive time per launch and touches only the
+    spare vector: the resident solution and a following solve are those of
+    a context that never ran it.  Config 1's mesh tunes to the direct
+    variant, for which the hook reports APHID_E_STATE; both contexts then
+    take the 64x4 march with 12-plane chunks."""
+    from aphid_git_amd import gpu as G
+
+    case = meshgen.config1()
+    with G.Solver() as s, G.Solver() as ref:
+        b = s.setup_case(case)
+        ref.setup_case(case)
+        s.time_apply(0, 1)  # tunes the tile
+        if s.get_tile(0)[2] == 0:
+            with pytest.raises(G.AphidError, match="direct variant"):
+                s.time_traffic(1)
+        s.set_tile(64, 4, 12)
+        ref.set_tile(64, 4, 12)
+        r1 = s.qmr_solve(b, 1e-8, 5000, hist_cap=0)
+        rr = ref.qmr_solve(b, 1e-8, 5000, hist_cap=0)
+        s.load_rhs(b)
+        ref.load_rhs(b)
+        a = s.solve_resident(1e-8, 5000)
+        t = s.time_traffic(5)
+        ta = s.time_apply(0, 5)
+        print(f"[traffic] config1: {t * 1e6:.2f} us per launch, apply {ta * 1e6:.2f} us", flush=True)
+        assert np.isfinite(t) and t > 0
+        x = np.concatenate([v.ravel() for v in s.fetch_solution()])
+        c = ref.solve_resident(1e-8, 5000)
+        xr = np.concatenate([v.ravel() for v in ref.fetch_solution()])
+        assert a.iters == c.iters and np.array_equal(x, xr)
+        assert r1.iters == rr.iters
