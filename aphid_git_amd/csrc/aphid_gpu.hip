@@ -87,12 +87,24 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
   }
   const int chunk = t.chunk;
   dim3 grid((unsigned)ntk * ntj * ((op.nx + chunk - 1) / chunk));
+  // reproducible eps partials (complex128 only; chunks multiples of 4)
+  if (ep.seg && (!std::is_same<R, double>::value || chunk % 4 != 0)) return hipErrorInvalidValue;
 #define APHID_TILE(TK, TJ)                                                                 \
   if (t.tk == TK && t.tj == TJ) {                                                          \
-    if (t.nt)                                                                              \
+    if (ep.seg) {                                                                          \
+      if constexpr (std::is_same<R, double>::value) {                                      \
+        if (t.nt)                                                                          \
+          hipLaunchKernelGGL((k_apply<R, TK, TJ, true, true>), grid, block, 0, s, op, in, out, \
+                             chunk, ep);                                                   \
+        else                                                                               \
+          hipLaunchKernelGGL((k_apply<R, TK, TJ, false, true>), grid, block, 0, s, op, in,  \
+                             out, chunk, ep);                                              \
+      }                                                                                    \
+    } else if (t.nt) {                                                                     \
       hipLaunchKernelGGL((k_apply<R, TK, TJ, true>), grid, block, 0, s, op, in, out, chunk, ep); \
-    else                                                                                   \
+    } else {                                                                               \
       hipLaunchKernelGGL((k_apply<R, TK, TJ, false>), grid, block, 0, s, op, in, out, chunk, ep); \
+    }                                                                                      \
     return hipGetLastError();                                                              \
   }
   APHID_TILE(16, 16)
@@ -248,6 +260,11 @@ struct aphid_ctx {
   std::chrono::steady_clock::time_point t_solve;
   std::vector<std::pair<int, double>> marks;
   void* scratch = nullptr;  // device address of `scratch` on this context's device
+  // reproducible complex128 solves (aphid_set_reproducible): eps from the
+  // stencil's (column, 4-plane segment) partials, seg_n pairs
+  bool repro = false;
+  double* seg = nullptr;
+  long long seg_n = 0;
 };
 
 namespace {
@@ -272,6 +289,9 @@ void free_mesh(aphid_ctx* c) {
   c->tab64 = nullptr; c->tab32 = nullptr; c->dmet = nullptr; c->sig = nullptr; c->k2 = nullptr;
   c->k2f = nullptr; c->diag = nullptr; c->invd = nullptr; c->invdf = nullptr; c->stage = nullptr;
   c->stagef = nullptr; c->hist = nullptr; c->hist_cap = 0;
+  if (c->seg) (void)hipFree(c->seg);
+  c->seg = nullptr;
+  c->seg_n = 0;
   c->have_k2 = c->have_diag = false;
   c->tuned[0] = c->tuned[1] = false;
   c->stuned[0] = c->stuned[1] = false;
@@ -413,7 +433,12 @@ bool tile_ok(const aphid_ctx* c, int tk, int tj, int chunk) {
 // named by APHID_TUNE_CACHE (if set), and in memory for the process.
 int autotune(aphid_ctx* c, int prec) {
   if (c->tuned[prec]) return 0;
-  const TuneKey tk0{c->dev, c->nx, c->ny, c->nz, prec};
+  // a reproducible complex128 context picks among x-march tiles with chunks
+  // that are multiples of 4 (its eps partials are per 4-plane segment), and
+  // keeps its choice apart from the plain one
+  const bool seg = prec == 0 && c->repro;
+  const int pk = prec + (seg ? 2 : 0);
+  const TuneKey tk0{c->dev, c->nx, c->ny, c->nz, pk};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     const auto it = g_tiles.find(tk0);
@@ -424,7 +449,7 @@ int autotune(aphid_ctx* c, int prec) {
     }
   }
   char key[96];
-  snprintf(key, sizeof key, "%d %d %d %d", c->nx, c->ny, c->nz, prec);
+  snprintf(key, sizeof key, "%d %d %d %d", c->nx, c->ny, c->nz, pk);
   const char* cache = getenv("APHID_TUNE_CACHE");
   if (cache) {
     if (FILE* f = fopen(cache, "r")) {
@@ -436,7 +461,7 @@ int autotune(aphid_ctx* c, int prec) {
           snprintf(k2, sizeof k2, "%d %d %d %d", a, b, cc, p);
           // the checks of aphid_set_tile: a line that fails them (stale or
           // edited) is ignored, and the tile is timed instead
-          if (!strcmp(k2, key) && tile_ok(c, tk, tj, ch)) {
+          if (!strcmp(k2, key) && tile_ok(c, tk, tj, ch) && (!seg || (ch > 0 && ch % 4 == 0))) {
             c->tile[prec] = {tk, tj, ch};
             c->tuned[prec] = true;
           }
@@ -460,6 +485,7 @@ int autotune(aphid_ctx* c, int prec) {
     for (int ch : chunks) {
       Tile t = {base.tk, base.tj, ch};
       if (apply_blocks(t, c->nx, c->ny, c->nz) > (unsigned)kMaxPartBlocks) continue;
+      if (seg && (ch == 0 || ch % 4 != 0)) continue;
       float ms = 0.f;
       if (prec == 0)
         rc = time_tile<double>(c, t, c->op64, c->V[3], c->V[4], reps, &ms);
@@ -526,6 +552,43 @@ template <>
 const OpArgs<double>& opargs<double>(aphid_ctx* c) { return c->op64; }
 template <>
 const OpArgs<float>& opargs<float>(aphid_ctx* c) { return c->op32; }
+
+// reproducible solves: the eps partials of every (column, 4-plane segment)
+int ensure_seg(aphid_ctx* c) {
+  if (!c->repro || c->seg) return 0;
+  c->seg_n = (long long)((c->nx + 3) / 4) * c->ny * c->nz;
+  CK(hipMalloc(&c->seg, sizeof(double) * 2 * c->seg_n));
+  return 0;
+}
+
+// The stencil with the eps = p^T K p epilogue and its finalize on state st
+// (phase B of the QMR iteration): per-block partials summed by one block,
+// or, for a reproducible complex128 solve, the stencil's per-segment
+// partials summed by a fixed grid (k_seg_reduce) and then by one block.
+// `mid` (optional): an event recorded right after the stencil.
+constexpr int kSegBlocks = 512;
+template <typename V>
+hipError_t launch_stencil_eps(aphid_ctx* c, const Tile& tl, const cplx<V>* p, cplx<V>* t,
+                              QmrState* st, hipEvent_t mid = nullptr) {
+  if (std::is_same<V, double>::value && c->repro) {
+    ApplyEpilogue ep{st, c->part, c->seg};
+    const hipError_t e = launch_apply<V>(tl, opargs<V>(c), p, t, ep, c->s);
+    if (e != hipSuccess) return e;
+    if (mid) (void)hipEventRecord(mid, c->s);
+    const long long per = (c->seg_n + kSegBlocks - 1) / kSegBlocks;
+    hipLaunchKernelGGL(k_seg_reduce, dim3(kSegBlocks), dim3(kBlock), 0, c->s, c->seg, c->seg_n,
+                       per, st, c->part);
+    hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part, kSegBlocks, st);
+    return hipGetLastError();
+  }
+  ApplyEpilogue ep{st, c->part};
+  const hipError_t e = launch_apply<V>(tl, opargs<V>(c), p, t, ep, c->s);
+  if (mid) (void)hipEventRecord(mid, c->s);
+  hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
+                     (int)apply_blocks(tl, c->nx, c->ny, c->nz), st);
+  return e;
+}
+
 
 template <typename V>
 const cplx<V>* invd_of(aphid_ctx* c);
@@ -615,6 +678,7 @@ template <typename V>
 int tune_streams(aphid_ctx* c) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
   if (c->stuned[prec]) return 0;
+  if (int rc = ensure_seg(c)) return rc;
   const TuneKey tk0{c->dev, c->nx, c->ny, c->nz, prec};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -691,13 +755,8 @@ int tune_streams(aphid_ctx* c) {
     switch (k) {
       case 0:
         return launch_pupdate<V>(sh[SK_PUPDATE], w.v, invd, w.p, n4, c->tst, c->s);
-      case 1: {
-        ApplyEpilogue ep{c->tst, c->part};
-        const hipError_t e = launch_apply<V>(tl, opargs<V>(c), w.p, w.t, ep, c->s);
-        hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
-                           (int)apply_blocks(tl, c->nx, c->ny, c->nz), c->tst);
-        return e;
-      }
+      case 1:
+        return launch_stencil_eps<V>(c, tl, w.p, w.t, c->tst);
       case 2:
         return launch_vtnorm<V>(sh[SK_VTNORM], w.t, w.v, n4, c->tst, c->part, c->s);
       default:
@@ -813,11 +872,7 @@ int enqueue_iteration(aphid_ctx* c) {
   const cplx<V>* invd = invd_of<V>(c);
   const Stream* sh = c->stream[prec];
   CK(launch_pupdate<V>(sh[SK_PUPDATE], w.v, invd, w.p, n4, c->st, c->s));
-  ApplyEpilogue ep{c->st, c->part};
-  const Tile& tl = c->tile[prec];
-  CK(launch_apply<V>(tl, opargs<V>(c), w.p, w.t, ep, c->s));
-  hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
-                     (int)apply_blocks(tl, c->nx, c->ny, c->nz), c->st);
+  CK(launch_stencil_eps<V>(c, c->tile[prec], w.p, w.t, c->st));
   CK(launch_vtnorm<V>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s));
   CK(launch_update<V>(sh[SK_UPDATE], w, invd, n4, c->st, c->part, c->hist, c->s));
   CK(hipGetLastError());
@@ -963,7 +1018,8 @@ int run_loop(aphid_ctx* c) {
 template <typename V>
 int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs) {
   constexpr int prec = std::is_same<V, float>::value ? 1 : 0;
-  int trc = tune_streams<V>(c);
+  int trc = ensure_seg(c);
+  if (!trc) trc = tune_streams<V>(c);
   if (trc) return trc;
   const Vecs<V> w = vecs<V>(c);
   const long long n4 = 4 * c->nn;
@@ -2089,6 +2145,8 @@ int aphid_get_tile(aphid_ctx* c, int precision, int* tk, int* tj, int* chunk) {
 int aphid_set_tile(aphid_ctx* c, int precision, int tk, int tj, int chunk) {
   if (!c || precision < 0 || precision > 1 || chunk < 0) return fail(APHID_E_ARG, "bad argument");
   if (!tile_ok(c, tk, tj, chunk)) return fail(APHID_E_ARG, "unsupported tile, or too many blocks");
+  if (precision == 0 && c->repro && (chunk == 0 || chunk % 4 != 0))
+    return fail(APHID_E_ARG, "a reproducible context needs an x-march chunk that is a multiple of 4");
   c->tile[precision] = {tk, tj, chunk};
   c->tuned[precision] = true;
   if (c->gexec[precision]) (void)hipGraphExecDestroy(c->gexec[precision]), c->gexec[precision] = nullptr;
@@ -2111,6 +2169,25 @@ int aphid_time_apply(aphid_ctx* c, int precision, int reps, double* seconds) {
     rc = time_tile<float>(c, c->tile[1], c->op32, c->F[3], c->F[4], reps, &ms);
   if (rc) return rc;
   *seconds = 1e-3 * ms;
+  return 0;
+}
+
+int aphid_set_reproducible(aphid_ctx* c, int on) {
+  if (!c) return fail(APHID_E_ARG, "NULL context");
+  const bool r = on != 0;
+  if (r == c->repro) return 0;
+  CK(hipSetDevice(c->dev));
+  c->repro = r;
+  // the iteration graph and the tile belong to the other mode
+  if (c->gexec[0]) (void)hipGraphExecDestroy(c->gexec[0]), c->gexec[0] = nullptr;
+  c->giters[0] = 0;
+  c->tuned[0] = false;
+  c->stuned[0] = false;
+  if (!r && c->seg) {
+    CK(hipFree(c->seg));
+    c->seg = nullptr;
+    c->seg_n = 0;
+  }
   return 0;
 }
 
@@ -2164,6 +2241,7 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   if (!c || !secs || reps < 1 || precision != 0) return fail(APHID_E_ARG, "bad argument");
   if (!c->have_k2 || !c->V[0]) return fail(APHID_E_STATE, "solve once first");
   CK(hipSetDevice(c->dev));
+  if (int rc = ensure_seg(c)) return rc;
   // continue the Krylov process of the last solve with the stop test off
   QmrState hs = *c->hst;
   hs.done = 0;
@@ -2182,12 +2260,7 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
     return launch_pupdate<double>(sh[SK_PUPDATE], w.v, c->invd, w.p, n4, c->st, c->s);
   };
   auto K1 = [&](hipEvent_t mid) {
-    ApplyEpilogue ep{c->st, c->part};
-    const hipError_t e = launch_apply<double>(c->tile[0], c->op64, w.p, w.t, ep, c->s);
-    if (mid) (void)hipEventRecord(mid, c->s);
-    hipLaunchKernelGGL(k_fin_eps, dim3(1), dim3(kBlock), 0, c->s, c->part,
-                       (int)apply_blocks(c->tile[0], c->nx, c->ny, c->nz), c->st);
-    return e;
+    return launch_stencil_eps<double>(c, c->tile[0], w.p, w.t, c->st, mid);
   };
   auto K2 = [&](hipEvent_t mid) {
     return launch_vtnorm<double>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s, mid);

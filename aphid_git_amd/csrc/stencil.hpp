@@ -49,6 +49,12 @@ struct OpArgs {
 struct ApplyEpilogue {
   QmrState* st;     // nullptr: plain apply
   double* part;     // per-block partials (2 doubles per block)
+  // reproducible reduction (x-march with chunks that are multiples of 4):
+  // instead of per-block partials, the partial of every (column, 4-plane
+  // x-segment), seg[2*(s*ny*nz + j*nz + k) + {0,1}], each summed in
+  // ascending i by the thread that owns the column; k_seg_reduce then sums
+  // them in a fixed order, so eps does not depend on the tile or chunk
+  double* seg = nullptr;
 };
 
 // 32-bit byte offsets from a base pointer: lets the compiler address every
@@ -266,7 +272,7 @@ struct MarchAcc {
   }
 };
 
-template <typename R, int TK, int TJ, bool NTS>
+template <typename R, int TK, int TJ, bool NTS, bool SEG = false>
 __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* __restrict__ in,
                                                   cplx<R>* __restrict__ out, int chunk,
                                                   ApplyEpilogue ep) {
@@ -392,6 +398,14 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
       sto_h<NTS>(out, 2 * nnb + ob, rz);
       sto_h<NTS>(out, 3 * nnb + ob, rp);
       if (ep.st) eps_acc<R>(ca, rx, ry, rz, rp, eacc_re, eacc_im);
+      if constexpr (SEG) {
+        if ((i & 3) == 3 || i == nx - 1) {
+          const size_t e = 2 * ((size_t)(i >> 2) * pl + (size_t)j * nz + k);
+          ep.seg[e] = eacc_re;
+          ep.seg[e + 1] = eacc_im;
+          eacc_re = eacc_im = 0.0;
+        }
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -404,11 +418,31 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     k2p_c = k2p_n;
   }
 
-  if (ep.st) {
-    double v[2] = {eacc_re, eacc_im};
-    block_sum<2>(v);
-    block_partial<2>(v, ep.part);
+  if constexpr (!SEG) {
+    if (ep.st) {
+      double v[2] = {eacc_re, eacc_im};
+      block_sum<2>(v);
+      block_partial<2>(v, ep.part);
+    }
   }
+}
+
+// The fixed-order sum of a reproducible apply's segment partials: block g
+// of a fixed grid sums entries [g*per, (g+1)*per) of the n (re, im) pairs,
+// each thread in ascending order at a stride of the block size, then the
+// block's fixed tree; the one-block finalize adds the grid's partials.
+// Nothing here depends on the stencil's launch shape.
+__global__ void k_seg_reduce(const double* __restrict__ seg, long long n, long long per,
+                             const QmrState* st, double* __restrict__ part) {
+  if (st->done) return;
+  double v[2] = {0.0, 0.0};
+  const long long a = (long long)blockIdx.x * per, b = min(n, a + per);
+  for (long long q = a + threadIdx.x; q < b; q += blockDim.x) {
+    v[0] += seg[2 * q];
+    v[1] += seg[2 * q + 1];
+  }
+  block_sum<2>(v);
+  block_partial<2>(v, part);
 }
 
 // ---------------------------------------------------------------------------

@@ -151,6 +151,13 @@
           type(aphid_qmr_report), intent(inout) :: rep
         end function
 
+        integer(c_int) function aphid_set_reproducible(ctx, on)       &
+                         bind(C, name='aphid_set_reproducible')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: on
+        end function
+
         integer(c_int) function aphid_load_rhs(ctx, bx, by, bz, bp,    &
                          x0x, x0y, x0z, x0p) bind(C, name='aphid_load_rhs')
           import :: c_int, c_ptr, c_double_complex

@@ -357,6 +357,11 @@ class Solver:
                "aphid_time_copy")
         return t.value
 
+    def set_reproducible(self, on: bool = True):
+        """Complex128 solves whose iterates do not depend on the stencil's
+        launch shape (eps from fixed-order segment partials)."""
+        _check(lib().aphid_set_reproducible(self._ctx, int(bool(on))), "aphid_set_reproducible")
+
     def time_traffic(self, reps: int = 20) -> float:
         """Seconds per launch of the tuned stencil tile's loads and stores
         alone (no arithmetic): the floor its access pattern sets."""
@@ -579,7 +584,7 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_set_cells", "aphid_set_sigma", "aphid_set_k2", "aphid_set_diag", "aphid_get_k2",
            "aphid_get_diag", "aphid_apply", "aphid_apply_noalloc", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
            "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
-           "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy", "aphid_time_traffic",
+           "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy", "aphid_time_traffic", "aphid_set_reproducible",
            "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",
            "aphid_batch_add_system", "aphid_batch_solve", "aphid_batch_get_solution",
            "aphid_batch_set_guess", "aphid_batch_set_sweep_guess", "aphid_batch_set_history",

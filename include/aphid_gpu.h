@@ -194,6 +194,21 @@ int aphid_set_tile(aphid_ctx *ctx, int precision, int tk, int tj, int chunk);
  * vectors and return the mean kernel time in seconds. */
 int aphid_time_apply(aphid_ctx *ctx, int precision, int reps, double *seconds);
 
+/* Reproducible complex128 solves (no counterpart in the reference, whose
+ * reduction order is fixed by its OpenMP thread count, src/aphid_qmr.f90:
+ * 441-972).  With on != 0 the stencil's eps = p^T K p partials are kept
+ * per (node column, 4-plane x-segment), each summed in ascending x by the
+ * thread that owns the column, and a fixed grid sums them in a fixed
+ * order; the iterates then no longer depend on the stencil's tile or chunk
+ * (every other reduction of the solve already runs on fixed grids), so the
+ * same inputs give bit-identical solves on any launch shape the tuner may
+ * pick, e.g. on GPUs with other CU counts.  The autotuner (and
+ * aphid_set_tile) then uses x-march tiles with chunks that are multiples
+ * of 4.  Costs a ny*nz*ceil(nx/4) pair buffer, its write in the stencil and
+ * one more kernel per iteration.  The mixed-precision path keeps per-block
+ * partials.  Default off. */
+int aphid_set_reproducible(aphid_ctx *ctx, int on);
+
 /* Mean seconds per launch of the stencil's memory traffic alone: the tuned
  * complex128 march tile's loads (own columns of the four fields and four
  * k^2 grids, halo cells) and stores (with its store hint) with no
