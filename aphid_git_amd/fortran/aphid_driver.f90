@@ -2,8 +2,10 @@
 ! aphid_driver.f90 -- program aphid_gpu_serial: the reference's serial  !
 ! flow (aphid_main, src/aphid.f90:6-118) with the solve on the GPU.     !
 !                                                                      !
-!   aphid_gpu_serial [-w dir]                                          !
+!   aphid_gpu_serial [-w dir] [-r]                                     !
 !                                                                      !
+! -r: reproducible solve (aphid_set_reproducible): the iterates do not  !
+! depend on the stencil launch shape the autotuner picks.               !
 ! Reads dir/aphid.input, dir/aphid.mesh and (mode 5) dir/aphid.source  !
 ! in the reference's formats (man/aphid.1:63-124), sets the mesh and    !
 ! the FP32 cell parameters on the device (read_mesh, :269-454), builds  !
@@ -45,9 +47,11 @@
       type(c_ptr) :: ctx, pa, pb, pc, pd
       type(aphid_qmr_report) :: rep
       integer :: nblk, b
+      logical :: repro
 
-!     -w <dir> (src/aphid.f90:47-60)
+!     -w <dir> (src/aphid.f90:47-60); -r reproducible solve
       wdir = '.'
+      repro = .false.
       nargs = command_argument_count()
       ia = 1
       do while (ia <= nargs)
@@ -56,6 +60,7 @@
           call get_command_argument(ia + 1, wdir)
           ia = ia + 2
         else
+          if (arg(1:2) == '-r') repro = .true.
           ia = ia + 1
         end if
       end do
@@ -109,6 +114,10 @@
 
       rc = aphid_ctx_create(ctx, -1_c_int)
       call check(rc, 'aphid_ctx_create')
+      if (repro) then
+        rc = aphid_set_reproducible(ctx, 1_c_int)
+        call check(rc, 'aphid_set_reproducible')
+      end if
       rc = aphid_set_mesh(ctx, int(nx, c_int), int(ny, c_int),          &
                           int(nz, c_int), xn, yn, zn)
       call check(rc, 'aphid_set_mesh')

@@ -79,3 +79,30 @@ def test_driver_solution_matches_oracle(tmp_path, oracle, model):
     assert len(prog) == len(it) // 10
     if prog:
         assert prog[0].split()[1:3] == ["10/", str(case.max_it)]
+
+
+@pytest.mark.gpu
+def test_driver_reproducible_flag(tmp_path, oracle):
+    """aphid_gpu_serial -r solves in reproducible mode: its solution equals,
+    bit for bit, a reproducible solve of the same case through the Python
+    binding in this process (whatever stencil tile each process's autotuner
+    picks), and the oracle's solve to 1e-7."""
+    _need_exe()
+    from aphid_git_amd import gpu as G
+
+    case = meshgen.tiny_case(13, 11, 9, seed=4, model="R", freq=50.0)
+    case.target, case.max_it = 1e-9, 3000
+    meshgen.write_case(case, str(tmp_path))
+    r = subprocess.run([EXE, "-w", str(tmp_path), "-r"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and "converged" in r.stdout, r.stdout + r.stderr
+    sol = np.fromfile(tmp_path / "aphid_solution.bin", dtype=np.complex128)
+    with G.Solver() as s:
+        b = s.setup_case(meshgen.read_case(str(tmp_path)))
+        s.set_reproducible(True)
+        s.set_tile(32, 8, 4)
+        rr = s.qmr_solve(b, case.target, case.max_it, hist_cap=0)
+    assert np.array_equal(sol, flat4(rr.x))
+    x, rep, _ = oracle.System(meshgen.read_case(str(tmp_path))).solve()
+    xo = flat4(x)
+    assert np.linalg.norm(sol - xo) / np.linalg.norm(xo) < 1e-7
