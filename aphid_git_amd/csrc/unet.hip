@@ -134,12 +134,35 @@ bool conv_lds() {
   return on;
 }
 
+// Blocks sharing one brick of a 3x3x3 layer: on the deep, small levels,
+// whose bricks alone leave most of the 256 CUs idle, the output-channel
+// groups are split over up to 4 blocks per brick (1 where there are
+// 384 bricks or more).
+int conv_split(int bricks, int nb) {
+  int sp = 1;
+  while (sp < 4 && nb % (2 * sp) == 0 && (long long)bricks * sp < 384) sp *= 2;
+  return sp;
+}
+
 #define CONV_CASE(KS, CI, CO, MODE)                                                     \
   if (ks == KS && cin == CI && cout == CO && mode == MODE) {                            \
     if (KS == 3 && conv_lds()) {                                                        \
       using Bk = Brick<CI, CO>;                                                         \
-      const dim3 gb((unsigned)brick_count(a.X, a.Y, a.Z, Bk::TX, Bk::TY));              \
-      hipLaunchKernelGGL((kc_conv3l<CI, CO, MODE>), gb, dim3(256), 0, s, a);            \
+      const int bricks = brick_count(a.X, a.Y, a.Z, Bk::TX, Bk::TY);                    \
+      const int sp = conv_split(bricks, Bk::NB);                                        \
+      if constexpr (Bk::NB % 4 == 0) {                                                  \
+        if (sp == 4) {                                                                  \
+          hipLaunchKernelGGL((kc_conv3l<CI, CO, MODE, 4>), dim3(bricks, 4), dim3(256), 0, s, a); \
+          return hipGetLastError();                                                     \
+        }                                                                               \
+      }                                                                                 \
+      if constexpr (Bk::NB % 2 == 0) {                                                  \
+        if (sp == 2) {                                                                  \
+          hipLaunchKernelGGL((kc_conv3l<CI, CO, MODE, 2>), dim3(bricks, 2), dim3(256), 0, s, a); \
+          return hipGetLastError();                                                     \
+        }                                                                               \
+      }                                                                                 \
+      hipLaunchKernelGGL((kc_conv3l<CI, CO, MODE>), dim3(bricks), dim3(256), 0, s, a);  \
     } else {                                                                            \
       hipLaunchKernelGGL((kc_conv<KS, CI, CO, MODE>), grid, dim3(256), 0, s, a);        \
     }                                                                                   \

@@ -156,10 +156,17 @@ inline int brick_count(int X, int Y, int Z, int tx, int ty) {
   return ((X + tx - 1) / tx) * ((Y + ty - 1) / ty) * ((Z + 15) / 16);
 }
 
-template <int CIN, int COUT, int MODE>
+// SPLIT > 1 (layers with fewer bricks than CUs): gridDim.y = SPLIT blocks
+// share a brick, each computing NB/SPLIT of its 16-channel output groups
+// (group blockIdx.y*NB/SPLIT + n); every output's sum runs in the same
+// order as with one block per brick.
+template <int CIN, int COUT, int MODE, int SPLIT = 1>
 __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
   using Bk = Brick<CIN, COUT>;
-  constexpr int NB = Bk::NB, MT = Bk::MT, TX = Bk::TX, TY = Bk::TY;
+  constexpr int NBT = Bk::NB, MT = Bk::MT, TX = Bk::TX, TY = Bk::TY;
+  static_assert(NBT % SPLIT == 0, "SPLIT must divide the output channel groups");
+  constexpr int NB = NBT / SPLIT;
+  const int nb0 = (int)blockIdx.y * NB;  // first output channel group of this block
   constexpr int BY = Bk::BY, BZ = Bk::BZ, NV = Bk::NV;
   constexpr int KPAD = cpad(27 * CIN, 32);
   constexpr int PCS = CIN / 8;  // 16-byte pieces per voxel
@@ -208,7 +215,7 @@ __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
   constexpr int NK = KPAD / 32;
   bf16x8 bw[NB], bwn[NB];
 #pragma unroll
-  for (int n = 0; n < NB; ++n) bw[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + 8 * g);
+  for (int n = 0; n < NB; ++n) bw[n] = ld8(a.w + (size_t)(16 * (nb0 + n) + r) * KPAD + 8 * g);
   for (int ks = 0; ks < NK; ++ks) {
     const int kk = ks * 32 + 8 * g;
     const int tap = kk / CIN, ci = kk - tap * CIN;
@@ -216,7 +223,7 @@ __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
     const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
     const int kn = ks + 1 < NK ? kk + 32 : kk;  // the last step reloads its own
 #pragma unroll
-    for (int n = 0; n < NB; ++n) bwn[n] = ld8(a.w + (size_t)(16 * n + r) * KPAD + kn);
+    for (int n = 0; n < NB; ++n) bwn[n] = ld8(a.w + (size_t)(16 * (nb0 + n) + r) * KPAD + kn);
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
       const int row = w * MT + t, tx = row / TY, ty = row - tx * TY;
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(256) void kc_conv3l(ConvArgs a) {
     if (x >= a.X || y >= a.Y) continue;
 #pragma unroll
     for (int n = 0; n < NB; ++n) {
-      const int co = 16 * n + r;
+      const int co = 16 * (nb0 + n) + r;
       if (co >= COUT) continue;
       const float bias = MODE == 2 ? 0.f : a.bias[co];
 #pragma unroll
