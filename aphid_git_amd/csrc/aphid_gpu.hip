@@ -67,6 +67,10 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
                         ApplyEpilogue ep, hipStream_t s) {
   const int ntk = (op.nz + t.tk - 1) / t.tk, ntj = (op.ny + t.tj - 1) / t.tj;
   dim3 block(t.tk * t.tj);
+  // reproducible eps partials: complex128 x-march with chunks that are
+  // multiples of 4 only
+  if (ep.seg && (!std::is_same<R, double>::value || t.chunk <= 0 || t.chunk % 4 != 0))
+    return hipErrorInvalidValue;
   if (t.chunk <= 0) {
     dim3 grid((unsigned)ntk * ntj * op.nx);
 #define APHID_TILE(TK, TJ)                                                                \
@@ -87,8 +91,6 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
   }
   const int chunk = t.chunk;
   dim3 grid((unsigned)ntk * ntj * ((op.nx + chunk - 1) / chunk));
-  // reproducible eps partials (complex128 only; chunks multiples of 4)
-  if (ep.seg && (!std::is_same<R, double>::value || chunk % 4 != 0)) return hipErrorInvalidValue;
 #define APHID_TILE(TK, TJ)                                                                 \
   if (t.tk == TK && t.tj == TJ) {                                                          \
     if (ep.seg) {                                                                          \
