@@ -603,13 +603,14 @@ const cplx<float>* invd_of<float>(aphid_ctx* c) { return c->invdf; }
 // the launch shape `sh` (the grid is also the number of partials).
 #define APHID_STREAM_SHAPES(X) X(2, false) X(2, true) X(4, false) X(4, true)
 
+// the p-update with the d and x updates of the previous iteration
 template <typename V>
-hipError_t launch_pupdate(const Stream& sh, const cplx<V>* v, const cplx<V>* invd, cplx<V>* p,
-                          long long n, const QmrState* st, hipStream_t s) {
+hipError_t launch_pupdate(const Stream& sh, const Vecs<V>& w, const cplx<V>* invd, long long n,
+                          QmrState* st, hipStream_t s) {
 #define X(U, NT)                                                                              \
   if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
-    hipLaunchKernelGGL((k_pupdate<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, v, invd, p, n, \
-                       st);                                                                   \
+    hipLaunchKernelGGL((k_pupdate<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, w.v, invd, w.p, \
+                       w.d, w.x, n, st);                                                      \
     return hipGetLastError();                                                                 \
   }
   APHID_STREAM_SHAPES(X)
@@ -640,8 +641,8 @@ hipError_t launch_update(const Stream& sh, const Vecs<V>& w, const cplx<V>* invd
                          hipEvent_t mid = nullptr) {
 #define X(U, NT)                                                                              \
   if (sh.u == U && (sh.nt != 0) == NT) {                                                      \
-    hipLaunchKernelGGL((k_update<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, w.p, w.d, w.x,  \
-                       w.t, w.s, w.r, w.v, invd, n, st, part);                                \
+    hipLaunchKernelGGL((k_update<V, U, NT>), dim3(sh.grid), dim3(kBlock), 0, s, w.t, w.s, w.r,  \
+                       w.v, invd, n, st, part);                                               \
     if (mid) (void)hipEventRecord(mid, s);                                                    \
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBlock), 0, s, part, sh.grid, st, hist);    \
     return hipGetLastError();                                                                 \
@@ -750,13 +751,13 @@ int tune_streams(aphid_ctx* c) {
   ts.c_nm = 1.0;
   ts.max_it = 1 << 30;
   ts.iter = 1;
-  // an even number of sweeps: the fused update alternates between
-  // iterations that store x and r and iterations that do not (k_update)
+  // an even number of sweeps: the p-update and the fused update alternate
+  // between iterations that store x and r and iterations that do not
   const int reps = c->nn > 2000000 ? 4 : 8;
   auto launch = [&](int k, const Stream (&sh)[SK_COUNT]) -> hipError_t {
     switch (k) {
       case 0:
-        return launch_pupdate<V>(sh[SK_PUPDATE], w.v, invd, w.p, n4, c->tst, c->s);
+        return launch_pupdate<V>(sh[SK_PUPDATE], w, invd, n4, c->tst, c->s);
       case 1:
         return launch_stencil_eps<V>(c, tl, w.p, w.t, c->tst);
       case 2:
@@ -770,7 +771,7 @@ int tune_streams(aphid_ctx* c) {
     float tot = 0.f;
     for (int r = 0; r <= reps; ++r)  // the first sweep warms up
       for (int k = 0; k < 4; ++k) {
-        ts.iter = 1 + (r & 1);
+        ts.iter = 2 + (r & 1);
         CK(hipMemcpyAsync(c->tst, &ts, sizeof ts, hipMemcpyHostToDevice, c->s));
         CK(hipEventRecord(c->ev0, c->s));
         CK(launch(k, sh));
@@ -873,7 +874,7 @@ int enqueue_iteration(aphid_ctx* c) {
   const long long n4 = 4 * c->nn;
   const cplx<V>* invd = invd_of<V>(c);
   const Stream* sh = c->stream[prec];
-  CK(launch_pupdate<V>(sh[SK_PUPDATE], w.v, invd, w.p, n4, c->st, c->s));
+  CK(launch_pupdate<V>(sh[SK_PUPDATE], w, invd, n4, c->st, c->s));
   CK(launch_stencil_eps<V>(c, c->tile[prec], w.p, w.t, c->st));
   CK(launch_vtnorm<V>(sh[SK_VTNORM], w.t, w.v, n4, c->st, c->part, c->s));
   CK(launch_update<V>(sh[SK_UPDATE], w, invd, n4, c->st, c->part, c->hist, c->s));
@@ -1051,11 +1052,12 @@ int qmr_device(aphid_ctx* c, double tol, int max_it, bool have_x0, double* secs)
   int rc = 0;
   if (!c->hst->done) rc = run_loop<V>(c);
   if (rc) return rc;
-  // the loop leaves c->hst current; an odd last iteration left x and r one
-  // step behind
-  if (c->hst->lag) {
-    hipLaunchKernelGGL(k_catch_up<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.x, w.d, w.r, w.s,
-                       n4);
+  // the loop leaves c->hst current: the last iteration's d and x updates
+  // (due in the p-update that did not run) and an odd last iteration's x
+  // and r (left one step behind)
+  if (c->hst->upd_it > 0) {
+    hipLaunchKernelGGL(k_finish<V>, dim3(kRedBlocks), dim3(kBlock), 0, c->s, w.x, w.d, w.p, w.r,
+                       w.s, n4, c->st);
     CK(hipGetLastError());
   }
   CK(hipEventRecord(c->ev1, c->s));
@@ -2248,7 +2250,7 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   QmrState hs = *c->hst;
   hs.done = 0;
   hs.status = ST_MAXIT;
-  hs.iter = 1;
+  hs.iter = 2;  // the p-update then carries the d and x updates, as in a solve
   hs.max_it = 1 << 30;
   hs.target = 0.0;
   if (!(hs.rho_np > 0.0) || !std::isfinite(hs.rho_np) || is_zero(hs.del))
@@ -2259,7 +2261,7 @@ int aphid_time_iteration(aphid_ctx* c, int precision, int reps, double* secs) {
   const Stream* sh = c->stream[0];
   // ev (optional): events recorded between a kernel and its finalize
   auto K0 = [&]() {
-    return launch_pupdate<double>(sh[SK_PUPDATE], w.v, c->invd, w.p, n4, c->st, c->s);
+    return launch_pupdate<double>(sh[SK_PUPDATE], w, c->invd, n4, c->st, c->s);
   };
   auto K1 = [&](hipEvent_t mid) {
     return launch_stencil_eps<double>(c, c->tile[0], w.p, w.t, c->st, mid);
@@ -2594,9 +2596,9 @@ void batch_launch(aphid_batch* B, const BatchDev& D, int k, const int* nts,
   switch (k) {
     case 0:
       if (nts[SK_PUPDATE])
-        hipLaunchKernelGGL((kb_pupdate<V, true>), gs, bs, 0, B->s, D, v, invd, p);
+        hipLaunchKernelGGL((kb_pupdate<V, true>), gs, bs, 0, B->s, D, v, invd, p, d, x);
       else
-        hipLaunchKernelGGL((kb_pupdate<V, false>), gs, bs, 0, B->s, D, v, invd, p);
+        hipLaunchKernelGGL((kb_pupdate<V, false>), gs, bs, 0, B->s, D, v, invd, p, d, x);
       break;
     case 1:
       batch_apply<V>(B, D, ops, p, t, 1);
@@ -2611,11 +2613,9 @@ void batch_launch(aphid_batch* B, const BatchDev& D, int k, const int* nts,
       break;
     default:
       if (nts[SK_UPDATE])
-        hipLaunchKernelGGL((kb_update<V, true>), gs, bs, 0, B->s, D, p, d, x, t, s, r, v, invd,
-                           B->part);
+        hipLaunchKernelGGL((kb_update<V, true>), gs, bs, 0, B->s, D, t, s, r, v, invd, B->part);
       else
-        hipLaunchKernelGGL((kb_update<V, false>), gs, bs, 0, B->s, D, p, d, x, t, s, r, v, invd,
-                           B->part);
+        hipLaunchKernelGGL((kb_update<V, false>), gs, bs, 0, B->s, D, t, s, r, v, invd, B->part);
       hipLaunchKernelGGL(kb_fin_update, dim3(ns), bs, 0, B->s, D, B->part);
   }
 }
@@ -2935,8 +2935,9 @@ int batch_qmr(aphid_batch* B, double tol, const double* itol, int max_it, bool u
     if (rc) return rc;
     mark();
     if (batch_all_done(B)) {
-      // systems that stopped after an odd iteration: x and r one step behind
-      hipLaunchKernelGGL(kb_catch_up<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, x, d, r, s_);
+      // every system's last d and x updates, and x and r left one step
+      // behind by an odd last iteration
+      hipLaunchKernelGGL(kb_finish<V>, dim3(B->nstream), dim3(kBlock), 0, B->s, D, x, d, p, r, s_);
       CK(hipGetLastError());
       return 0;
     }

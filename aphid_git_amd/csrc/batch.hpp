@@ -191,17 +191,36 @@ __global__ void kb_fin_init_v(BatchDev B, const double* part, int max_it, int hi
 // 1/diag loads stay plain, as in the single-system kernels.
 template <typename V, bool NTS = false>
 __global__ void kb_pupdate(BatchDev B, const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
-                           vec<V>* __restrict__ p) {
+                           vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x) {
   const BItem it = b_item(B);
-  const QmrState* st = &B.st[it.s];
+  QmrState* st = &B.st[it.s];
   if (st->done) return;
-  const c128 k1 = st->k1;
+  const c128 k1 = st->k1, eta = st->eta;
+  const double k2 = st->k2s;
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
+  // the d and x updates of iteration m = iter - 1 of this system, as in
+  // k_pupdate (x stored for even m only; none for m = 0)
+  const int m = st->iter - 1;
   struct L {
-    vec<V> v, i, p;
+    vec<V> v, i, p, d, x;
   };
-  b_loop2(it, [&](long long q) { return L{ldv<NTS>(v + q), ldv<false>(invd + q + io), ldv<NTS>(p + q)}; },
-          [&](long long q, const L& l) { stv<NTS>(p + q, dn<V>(up(l.v) * up(l.i) - up(l.p) * k1)); });
+  auto load = [&](long long q, int mode) {
+    return L{ldv<NTS>(v + q), ldv<false>(invd + q + io), ldv<NTS>(p + q),
+             mode != PU_P ? ldv<NTS>(d + q) : vec<V>{}, mode == PU_PDX ? ldv<NTS>(x + q) : vec<V>{}};
+  };
+  if (m <= 0)
+    b_loop2(it, [&](long long q) { return load(q, PU_P); }, [&](long long q, const L& l) {
+      pupdate_elem<V, NTS, PU_P>(p, d, x, q, l.v, l.i, l.p, l.d, l.x, k1, eta, k2);
+    });
+  else if (m & 1)
+    b_loop2(it, [&](long long q) { return load(q, PU_PD); }, [&](long long q, const L& l) {
+      pupdate_elem<V, NTS, PU_PD>(p, d, x, q, l.v, l.i, l.p, l.d, l.x, k1, eta, k2);
+    });
+  else
+    b_loop2(it, [&](long long q) { return load(q, PU_PDX); }, [&](long long q, const L& l) {
+      pupdate_elem<V, NTS, PU_PDX>(p, d, x, q, l.v, l.i, l.p, l.d, l.x, k1, eta, k2);
+    });
+  if ((int)blockIdx.x == B.bstart[it.s] && threadIdx.x == 0) st->dx_done = m > 0 ? m : 0;
 }
 
 // batched stencil, x-march over flat in-plane chunks.  A block owns NT
@@ -444,8 +463,7 @@ __global__ void kb_fin_vtnorm(BatchDev B, const double* part) {
 }
 
 template <typename V, bool NTS = false>
-__global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __restrict__ d,
-                          vec<V>* __restrict__ x, const vec<V>* __restrict__ t, vec<V>* __restrict__ s_,
+__global__ void kb_update(BatchDev B, const vec<V>* __restrict__ t, vec<V>* __restrict__ s_,
                           vec<V>* __restrict__ r, vec<V>* __restrict__ v,
                           const vec<V>* __restrict__ invd, double* part) {
   const BItem it = b_item(B);
@@ -454,42 +472,58 @@ __global__ void kb_update(BatchDev B, const vec<V>* __restrict__ p, vec<V>* __re
   const c128 eta = st->eta, beta = st->beta;
   const double k2 = st->k2s, rho = st->rho_np;
   const long long io = B.opoff[B.op[it.s]] - B.off[it.s];
-  // x and r are stored on even iterations only, as in k_update
+  // r is stored on even iterations only, as in k_update
   const bool full = (st->iter & 1) == 0;
   double a[3] = {0.0, 0.0, 0.0};
   struct L {
-    vec<V> p, d, x, t, s, r, v, i;
+    vec<V> t, s, r, v, i;
   };
-  auto load = [&](long long q, bool with_x) {
-    return L{ldv<NTS>(p + q), ldv<NTS>(d + q), with_x ? ldv<NTS>(x + q) : vec<V>{},
-             ldv<NTS>(t + q), ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
+  auto load = [&](long long q) {
+    return L{ldv<NTS>(t + q), ldv<NTS>(s_ + q), ldv<NTS>(r + q), ldv<NTS>(v + q),
              ldv<false>(invd + q + io)};
   };
   if (full)
-    b_loop2(it, [&](long long q) { return load(q, true); }, [&](long long q, const L& l) {
-      update_elem<V, NTS, UPD_CATCHUP>(q, l.p, l.d, l.x, l.t, l.s, l.r, l.v, l.i, eta, beta, k2,
-                                       rho, d, x, s_, r, v, a);
+    b_loop2(it, load, [&](long long q, const L& l) {
+      update_elem<V, NTS, UPD_CATCHUP>(q, l.t, l.s, l.r, l.v, l.i, eta, beta, k2, rho, s_, r, v, a);
     });
   else
-    b_loop2(it, [&](long long q) { return load(q, false); }, [&](long long q, const L& l) {
-      update_elem<V, NTS, UPD_SKIP>(q, l.p, l.d, l.x, l.t, l.s, l.r, l.v, l.i, eta, beta, k2, rho,
-                                    d, x, s_, r, v, a);
+    b_loop2(it, load, [&](long long q, const L& l) {
+      update_elem<V, NTS, UPD_SKIP>(q, l.t, l.s, l.r, l.v, l.i, eta, beta, k2, rho, s_, r, v, a);
     });
-  if ((int)blockIdx.x == B.bstart[it.s] && threadIdx.x == 0) st->lag = full ? 0 : 1;
+  if ((int)blockIdx.x == B.bstart[it.s] && threadIdx.x == 0) {
+    st->lag = full ? 0 : 1;
+    st->upd_it = st->iter;
+  }
   block_sum<3>(a);
   block_partial<3>(a, part);
 }
 
-// x += d, r -= s for the systems whose last iteration left them one step
-// behind (kb_update)
+// Completes every system of a batch as k_finish completes a single solve:
+// the d and x updates its last p-update did not reach, x left one step
+// behind, and r left one step behind, from the system's own flags.
 template <typename V>
-__global__ void kb_catch_up(BatchDev B, vec<V>* __restrict__ x, const vec<V>* __restrict__ d,
-                            vec<V>* __restrict__ r, const vec<V>* __restrict__ s_) {
+__global__ void kb_finish(BatchDev B, vec<V>* __restrict__ x, vec<V>* __restrict__ d,
+                          const vec<V>* __restrict__ p, vec<V>* __restrict__ r,
+                          const vec<V>* __restrict__ s_) {
   const BItem it = b_item(B);
-  if (!B.st[it.s].lag) return;
+  const QmrState* st = &B.st[it.s];
+  const int pend = st->upd_it > st->dx_done ? st->upd_it : 0;
+  const bool xlag = !pend && (st->dx_done & 1);
+  const bool rlag = st->upd_it > 0 && st->lag;
+  if (!pend && !xlag && !rlag) return;
+  const c128 eta = st->eta;
+  const double k2 = st->k2s;
   B_LOOP(q, it) {
-    x[q] = dn<V>(up(x[q]) + up(d[q]));
-    r[q] = dn<V>(up(r[q]) - up(s_[q]));
+    if (pend) {
+      const vec<V> dq = d[q];
+      const vec<V> ddv = dn<V>(up(p[q]) * eta + up(dq) * k2);
+      d[q] = ddv;
+      x[q] = (pend & 1) ? dn<V>(up(x[q]) + up(ddv))
+                        : dn<V>(up(dn<V>(up(x[q]) + up(dq))) + up(ddv));
+    } else if (xlag) {
+      x[q] = dn<V>(up(x[q]) + up(d[q]));
+    }
+    if (rlag) r[q] = dn<V>(up(r[q]) - up(s_[q]));
   }
 }
 

@@ -2,12 +2,18 @@
 // (src/aphid_qmr.f90:297-390), templated on the storage precision V of the
 // Krylov vectors (double: the plain complex128 path; float: the complex64
 // path).  Element arithmetic, inner products and all recurrence scalars are
-// FP64.  One iteration is four launches:
-//   A  p = M^-1 v - k1 p                               (:306-320)
+// FP64.  One iteration n is four launches:
+//   A  d, x updates of iteration n-1 (:347-352), then p = M^-1 v - k1 p
+//      (:306-320)
 //   B  t = K p, eps = p^T t, beta = eps/del            (stencil.hpp, :324-330)
 //   C  rho' = ||t - beta v||, theta, c, eta, k2        (:332-345)
-//   D  d, x, s, r, v updates, ||r||, stop test and the next del = v^T M^-1 v
-//      (:347-390 and :306-313 of the next iteration)
+//   D  s, r, v updates, ||r||, stop test and the next del = v^T M^-1 v
+//      (:353-390 and :306-313 of the next iteration)
+// d_n = eta_n p_n + k2_n d_{n-1} needs the p_n that A of iteration n+1
+// reads anyway before overwriting it, and nothing reads d or x between D
+// of iteration n and A of iteration n+1, so doing them there reads p once
+// per iteration instead of twice; the arithmetic of every element is the
+// same.  k_finish completes d and x after the last iteration.
 // Every kernel returns at once when the state says the solve is done, so a
 // captured graph of several iterations can be replayed without a host
 // decision per iteration.
@@ -154,26 +160,71 @@ __device__ __forceinline__ void stv(cplx<T>* p, cplx<T> v) {
   }
 }
 
-// A: p = M^-1 v - p k1 (:306-320)
-template <typename V, int U = 4, bool NT = false>
-__global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
-                          vec<V>* __restrict__ p, long long n, const QmrState* st) {
-  if (st->done) return;
-  const c128 k1 = st->k1;
+// A of iteration m+1: d_m = eta_m p_m + k2_m d_{m-1} and x_m (:347-352),
+// then p = M^-1 v - p k1 (:306-320).  x is stored on even m only: an odd m
+// leaves x_{m-1} in memory, and the even m after it stores
+// x_m = (x_{m-2} + d_{m-1}) + d_m from the d_{m-1} it reads anyway, with
+// the operations that storing x_{m-1} would have taken, so every value is
+// bit-for-bit the one of storing x each time.  m = 0 (the first
+// iteration of a solve) has no d or x update.
+enum PupMode { PU_P, PU_PD, PU_PDX };
+template <typename V, bool NT, int MODE>
+__device__ __forceinline__ void pupdate_elem(vec<V>* __restrict__ p, vec<V>* __restrict__ d,
+                                             vec<V>* __restrict__ x, long long q, vec<V> vq,
+                                             vec<V> iq, vec<V> pq, vec<V> dq, vec<V> xq, c128 k1,
+                                             c128 eta, double k2) {
+  if constexpr (MODE != PU_P) {
+    const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
+    stv<NT>(d + q, ddv);
+    if constexpr (MODE == PU_PDX) stv<NT>(x + q, dn<V>(up(dn<V>(up(xq) + up(dq))) + up(ddv)));
+  }
+  stv<NT>(p + q, dn<V>(up(vq) * up(iq) - up(pq) * k1));
+}
+
+template <typename V, int U, bool NT, int MODE>
+__device__ __forceinline__ void pupdate_pass(const vec<V>* __restrict__ v,
+                                             const vec<V>* __restrict__ invd, vec<V>* __restrict__ p,
+                                             vec<V>* __restrict__ d, vec<V>* __restrict__ x,
+                                             long long n, const QmrState* st) {
+  const c128 k1 = st->k1, eta = st->eta;
+  const double k2 = st->k2s;
+  const vec<V> none{};
   UNROLLED_LOOP(U, q, stride, n) {
-    vec<V> a[U], b[U], c[U];
+    vec<V> a[U], b[U], c[U], dd[U], xx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      a[u] = ldv<NT>(v + q + u * stride);
-      b[u] = ldv<false>(invd + q + u * stride);
-      c[u] = ldv<NT>(p + q + u * stride);
+      const long long e = q + u * stride;
+      a[u] = ldv<NT>(v + e);
+      b[u] = ldv<false>(invd + e);
+      c[u] = ldv<NT>(p + e);
+      dd[u] = MODE != PU_P ? ldv<NT>(d + e) : none;
+      xx[u] = MODE == PU_PDX ? ldv<NT>(x + e) : none;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      stv<NT>(p + q + u * stride, dn<V>(up(a[u]) * up(b[u]) - up(c[u]) * k1));
+      pupdate_elem<V, NT, MODE>(p, d, x, q + u * stride, a[u], b[u], c[u], dd[u], xx[u], k1, eta, k2);
   }
   for (; q < n; q += stride)
-    stv<NT>(p + q, dn<V>(up(ldv<NT>(v + q)) * up(ldv<false>(invd + q)) - up(ldv<NT>(p + q)) * k1));
+    pupdate_elem<V, NT, MODE>(p, d, x, q, ldv<NT>(v + q), ldv<false>(invd + q), ldv<NT>(p + q),
+                              MODE != PU_P ? ldv<NT>(d + q) : none,
+                              MODE == PU_PDX ? ldv<NT>(x + q) : none, k1, eta, k2);
+}
+
+template <typename V, int U = 4, bool NT = false>
+__global__ void k_pupdate(const vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
+                          vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x,
+                          long long n, QmrState* st) {
+  if (st->done) return;
+  // iteration st->iter starts here; m = st->iter - 1 is the one whose d and
+  // x are due (its fused update ran, or m = 0)
+  const int m = st->iter - 1;
+  if (m <= 0)
+    pupdate_pass<V, U, NT, PU_P>(v, invd, p, d, x, n, st);
+  else if (m & 1)
+    pupdate_pass<V, U, NT, PU_PD>(v, invd, p, d, x, n, st);
+  else
+    pupdate_pass<V, U, NT, PU_PDX>(v, invd, p, d, x, n, st);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->dx_done = m > 0 ? m : 0;
 }
 
 // C: rho' = ||t - beta v|| and the step-4 scalars (:332-345)
@@ -221,29 +272,23 @@ __global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
   if (threadIdx.x == 0) vt_scalars(st, sqrt(tot[0]));
 }
 
-// one element of step D: d, x, s, r, v updates; accumulates ||r||^2 and
-// v^T M^-1 v (src/aphid_qmr.f90:347-369, 306-311).
-// x and r are stored every second iteration: an odd iteration of a solve
-// (UPD_SKIP) leaves them one step behind (x_{k-1}, r_{k-1} stay in memory;
-// r_k is formed for its norm only), and the even one after it
-// (UPD_CATCHUP) first rebuilds x_{k-1} = x_{k-2} + d_{k-1} and
-// r_{k-1} = r_{k-2} - s_{k-1} in registers from the d and s it reads
-// anyway, with the operations storing them would have taken, so every
-// value is bit-for-bit the one of storing x and r each time.  A solve that
-// stops after an odd iteration is completed by k_catch_up (kb_catch_up in
-// a batch).
+// one element of step D: s, r, v updates; accumulates ||r||^2 and
+// v^T M^-1 v (src/aphid_qmr.f90:353-369, 306-311).
+// r is stored every second iteration: an odd iteration of a solve
+// (UPD_SKIP) leaves it one step behind (r_{k-1} stays in memory; r_k is
+// formed for its norm only), and the even one after it (UPD_CATCHUP)
+// first rebuilds r_{k-1} = r_{k-2} - s_{k-1} in registers from the s it
+// reads anyway, with the operation storing it would have taken, so every
+// value is bit-for-bit the one of storing r each time.  A solve that stops
+// after an odd iteration is completed by k_finish (kb_finish in a batch).
 enum UpdMode { UPD_SKIP, UPD_CATCHUP };
 template <typename V, bool NT, int MODE>
-__device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_,
-                                            vec<V> sq, vec<V> rq, vec<V> vq, vec<V> iq, c128 eta,
-                                            c128 beta, double k2, double rho,
-                                            vec<V>* __restrict__ d, vec<V>* __restrict__ x,
-                                            vec<V>* __restrict__ s, vec<V>* __restrict__ r,
-                                            vec<V>* __restrict__ v, double (&a)[3]) {
+__device__ __forceinline__ void update_elem(long long q, vec<V> tq_, vec<V> sq, vec<V> rq,
+                                            vec<V> vq, vec<V> iq, c128 eta, c128 beta, double k2,
+                                            double rho, vec<V>* __restrict__ s,
+                                            vec<V>* __restrict__ r, vec<V>* __restrict__ v,
+                                            double (&a)[3]) {
   const c128 tq = up(tq_);
-  const vec<V> ddv = dn<V>(up(pq) * eta + up(dq) * k2);
-  stv<NT>(d + q, ddv);
-  if (MODE == UPD_CATCHUP) stv<NT>(x + q, dn<V>(up(dn<V>(up(xq) + up(dq))) + up(ddv)));
   const vec<V> ssv = dn<V>(tq * eta + up(sq) * k2);
   stv<NT>(s + q, ssv);
   const vec<V> rprev = MODE == UPD_CATCHUP ? dn<V>(up(rq) - up(sq)) : rq;
@@ -259,69 +304,84 @@ __device__ __forceinline__ void update_elem(long long q, vec<V> pq, vec<V> dq, v
   a[2] += pr.im;
 }
 
-// D: d = eta p + k2 d; x += d; s = eta t + k2 s; r -= s; v = (t - beta v)/rho';
-// ||r|| and the stop test (:347-390), then del = v^T M^-1 v and k1 for the
-// next iteration (:306-313).  x and r are stored on even iterations only
-// (update_elem); st->lag records whether this iteration left them behind.
+// D: s = eta t + k2 s; r -= s; v = (t - beta v)/rho'; ||r|| and the stop
+// test (:353-390), then del = v^T M^-1 v and k1 for the next iteration
+// (:306-313).  r is stored on even iterations only (update_elem);
+// st->lag records whether this iteration left it behind.
 template <typename V, int U, bool NT, int MODE>
-__device__ __forceinline__ void update_pass(const vec<V>* __restrict__ p, vec<V>* __restrict__ d,
-                                            vec<V>* __restrict__ x, const vec<V>* __restrict__ t,
-                                            vec<V>* __restrict__ s, vec<V>* __restrict__ r,
-                                            vec<V>* __restrict__ v, const vec<V>* __restrict__ invd,
-                                            long long n, const QmrState* st, double (&a)[3]) {
+__device__ __forceinline__ void update_pass(const vec<V>* __restrict__ t, vec<V>* __restrict__ s,
+                                            vec<V>* __restrict__ r, vec<V>* __restrict__ v,
+                                            const vec<V>* __restrict__ invd, long long n,
+                                            const QmrState* st, double (&a)[3]) {
   const c128 eta = st->eta, beta = st->beta;
   const double k2 = st->k2s, rho = st->rho_np;
-  auto body = [&](long long q, vec<V> pq, vec<V> dq, vec<V> xq, vec<V> tq_, vec<V> sq, vec<V> rq,
-                  vec<V> vq, vec<V> iq) {
-    update_elem<V, NT, MODE>(q, pq, dq, xq, tq_, sq, rq, vq, iq, eta, beta, k2, rho, d, x, s, r, v,
-                             a);
-  };
-  const vec<V> none{};
   UNROLLED_LOOP(U, q, stride, n) {
-    vec<V> P_[U], D_[U], X_[U], T_[U], S_[U], R_[U], Vv[U], I_[U];
+    vec<V> T_[U], S_[U], R_[U], Vv[U], I_[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long long e = q + u * stride;
-      P_[u] = ldv<NT>(p + e); D_[u] = ldv<NT>(d + e);
-      X_[u] = MODE != UPD_SKIP ? ldv<NT>(x + e) : none;
       T_[u] = ldv<NT>(t + e); S_[u] = ldv<NT>(s + e); R_[u] = ldv<NT>(r + e);
       Vv[u] = ldv<NT>(v + e); I_[u] = ldv<false>(invd + e);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) body(q + u * stride, P_[u], D_[u], X_[u], T_[u], S_[u], R_[u], Vv[u], I_[u]);
+    for (int u = 0; u < U; ++u)
+      update_elem<V, NT, MODE>(q + u * stride, T_[u], S_[u], R_[u], Vv[u], I_[u], eta, beta, k2, rho,
+                               s, r, v, a);
   }
   for (; q < n; q += stride)
-    body(q, ldv<NT>(p + q), ldv<NT>(d + q), MODE != UPD_SKIP ? ldv<NT>(x + q) : none, ldv<NT>(t + q),
-         ldv<NT>(s + q), ldv<NT>(r + q), ldv<NT>(v + q), ldv<false>(invd + q));
+    update_elem<V, NT, MODE>(q, ldv<NT>(t + q), ldv<NT>(s + q), ldv<NT>(r + q), ldv<NT>(v + q),
+                             ldv<false>(invd + q), eta, beta, k2, rho, s, r, v, a);
 }
 
 template <typename V, int U = 2, bool NT = false>
-__global__ void k_update(const vec<V>* __restrict__ p, vec<V>* __restrict__ d, vec<V>* __restrict__ x,
-                         const vec<V>* __restrict__ t, vec<V>* __restrict__ s, vec<V>* __restrict__ r,
+__global__ void k_update(const vec<V>* __restrict__ t, vec<V>* __restrict__ s, vec<V>* __restrict__ r,
                          vec<V>* __restrict__ v, const vec<V>* __restrict__ invd, long long n,
                          QmrState* st, double* part) {
   if (st->done) return;
-  // iteration 1 of a solve is odd: x0 and r0 are in memory
+  // iteration 1 of a solve is odd: r0 is in memory
   const bool full = (st->iter & 1) == 0;
   double a[3] = {0.0, 0.0, 0.0};
   if (full)
-    update_pass<V, U, NT, UPD_CATCHUP>(p, d, x, t, s, r, v, invd, n, st, a);
+    update_pass<V, U, NT, UPD_CATCHUP>(t, s, r, v, invd, n, st, a);
   else
-    update_pass<V, U, NT, UPD_SKIP>(p, d, x, t, s, r, v, invd, n, st, a);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->lag = full ? 0 : 1;
+    update_pass<V, U, NT, UPD_SKIP>(t, s, r, v, invd, n, st, a);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->lag = full ? 0 : 1;
+    st->upd_it = st->iter;
+  }
   block_sum<3>(a);
   block_partial<3>(a, part);
 }
 
-// completes x and r after a solve whose last iteration left them one step
-// behind: x += d, r -= s, as that iteration would have stored them
+// Completes a solve: the d and x updates of the last iteration whose fused
+// update ran, if the p-update of the next one did not run (x_m as the
+// p-update would have stored it, and stored for odd m too), x_m = x_{m-1}
+// + d_m if the last p-update left x behind (odd m), and r -= s if the last
+// update left r behind, all with the operations of the kernels they stand
+// in for.  The flags come from the device state, so one launch after the
+// loop suffices.
 template <typename V>
-__global__ void k_catch_up(vec<V>* __restrict__ x, const vec<V>* __restrict__ d,
-                           vec<V>* __restrict__ r, const vec<V>* __restrict__ s, long long n) {
+__global__ void k_finish(vec<V>* __restrict__ x, vec<V>* __restrict__ d,
+                         const vec<V>* __restrict__ p, vec<V>* __restrict__ r,
+                         const vec<V>* __restrict__ s, long long n, const QmrState* st) {
+  const int pend = st->upd_it > st->dx_done ? st->upd_it : 0;
+  const bool xlag = !pend && (st->dx_done & 1);
+  const bool rlag = st->upd_it > 0 && st->lag;
+  const c128 eta = st->eta;
+  const double k2 = st->k2s;
+  if (!pend && !xlag && !rlag) return;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x) {
-    x[q] = dn<V>(up(x[q]) + up(d[q]));
-    r[q] = dn<V>(up(r[q]) - up(s[q]));
+    if (pend) {
+      const vec<V> dq = d[q];
+      const vec<V> ddv = dn<V>(up(p[q]) * eta + up(dq) * k2);
+      d[q] = ddv;
+      x[q] = (pend & 1) ? dn<V>(up(x[q]) + up(ddv))
+                        : dn<V>(up(dn<V>(up(x[q]) + up(dq))) + up(ddv));
+    } else if (xlag) {
+      x[q] = dn<V>(up(x[q]) + up(d[q]));
+    }
+    if (rlag) r[q] = dn<V>(up(r[q]) - up(s[q]));
   }
 }
 // ||r||, history, stop test and counter (src/aphid_qmr.f90:371-390), then

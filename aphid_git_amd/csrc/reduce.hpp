@@ -76,7 +76,11 @@ struct QmrState {
   double rnorm, rhsnorm, target;
   int iter, max_it, status, done;
   int hist_cap;
-  int lag;  // 1: the last update left x and r one iteration behind (k_update)
+  int lag;  // 1: the last update left r one iteration behind (k_update)
+  // the d and x updates of iteration m run in the p-update of iteration
+  // m+1 (k_pupdate): upd_it = the last iteration whose fused update ran,
+  // dx_done = the last iteration whose d and x updates ran
+  int upd_it, dx_done;
 };
 
 enum : int { ST_CONV = 0, ST_MAXIT = 1, ST_BREAK = 2 };

@@ -46,11 +46,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Vector passes (reads + writes of one 4*nn complex vector) of a QMR
-# iteration: p-update 4, stencil 3 (u, k^2, K u), ||t - beta v|| 2, fused
-# update 11.5 on average (13 on even iterations, which store x and r; 10 on
-# odd ones, which leave them one step behind).
-UPDATE_PASSES = 11.5
-ITERATION_PASSES = 4 + 3 + 2 + UPDATE_PASSES
+# iteration as the library schedules it: p-update 7 on average (v, 1/diag,
+# p read and p written: 4; the previous iteration's d update: 2; its x
+# update, every second iteration: 2), stencil 3 (u, k^2, K u),
+# ||t - beta v|| 2, fused update 7.5 on average (t, s, r, v, 1/diag read,
+# s and v written; r written every second iteration).  The reference's
+# statements make 22 passes per iteration.
+PUPDATE_PASSES = 7.0
+UPDATE_PASSES = 7.5
+ITERATION_PASSES = PUPDATE_PASSES + 3 + 2 + UPDATE_PASSES
 BF16_PEAK_TFLOPS = 2500.0
 TOL = 1e-8
 MAX_IT = 100000
@@ -553,7 +557,7 @@ def main():
         s_bytes = 16
         B_mv = 3 * M * s_bytes
         per_kernel = {
-            "p_update": (4 * M * s_bytes, kt[0]),
+            "p_update": (PUPDATE_PASSES * M * s_bytes, kt[0]),
             "stencil_eps": (B_mv, kt[1]),
             "vt_norm": (2 * M * s_bytes, kt[2]),
             "fused_update": (UPDATE_PASSES * M * s_bytes, kt[3]),
