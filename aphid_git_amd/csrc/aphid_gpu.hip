@@ -62,6 +62,30 @@ constexpr Stream kStreamDefault[SK_COUNT] = {{4, 2048, 0}, {4, 2048, 0}, {2, 204
 const Tile kTiles[] = {{16, 16, 0}, {32, 8, 0}, {64, 4, 0}, {16, 8, 0}, {32, 4, 0},
                        {32, 16, 0}, {64, 8, 0}, {16, 32, 0}};
 
+// the x-march stencil of one tile: store hint (nt), reproducible eps
+// partials (ep.seg) and, for complex128, the compact k^2 (op.k2im; not with
+// the segment partials, whose registers it would spill)
+template <typename R, int TK, int TJ>
+hipError_t launch_march(dim3 grid, dim3 block, hipStream_t s, const OpArgs<R>& op,
+                        const cplx<R>* in, cplx<R>* out, int chunk, ApplyEpilogue ep, bool nt) {
+#define APHID_MARCH(NT, SEG, CKV)                                                            \
+  hipLaunchKernelGGL((k_apply<R, TK, TJ, NT, SEG, CKV>), grid, block, 0, s, op, in, out, chunk, ep)
+  if constexpr (std::is_same<R, double>::value) {
+    const bool ck = op.k2im != nullptr;
+    if (ep.seg) {
+      if (nt) APHID_MARCH(true, true, false); else APHID_MARCH(false, true, false);
+    } else if (ck) {
+      if (nt) APHID_MARCH(true, false, true); else APHID_MARCH(false, false, true);
+    } else {
+      if (nt) APHID_MARCH(true, false, false); else APHID_MARCH(false, false, false);
+    }
+  } else {
+    if (nt) APHID_MARCH(true, false, false); else APHID_MARCH(false, false, false);
+  }
+#undef APHID_MARCH
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, cplx<R>* out,
                         ApplyEpilogue ep, hipStream_t s) {
@@ -92,23 +116,8 @@ hipError_t launch_apply(const Tile& t, const OpArgs<R>& op, const cplx<R>* in, c
   const int chunk = t.chunk;
   dim3 grid((unsigned)ntk * ntj * ((op.nx + chunk - 1) / chunk));
 #define APHID_TILE(TK, TJ)                                                                 \
-  if (t.tk == TK && t.tj == TJ) {                                                          \
-    if (ep.seg) {                                                                          \
-      if constexpr (std::is_same<R, double>::value) {                                      \
-        if (t.nt)                                                                          \
-          hipLaunchKernelGGL((k_apply<R, TK, TJ, true, true>), grid, block, 0, s, op, in, out, \
-                             chunk, ep);                                                   \
-        else                                                                               \
-          hipLaunchKernelGGL((k_apply<R, TK, TJ, false, true>), grid, block, 0, s, op, in,  \
-                             out, chunk, ep);                                              \
-      }                                                                                    \
-    } else if (t.nt) {                                                                     \
-      hipLaunchKernelGGL((k_apply<R, TK, TJ, true>), grid, block, 0, s, op, in, out, chunk, ep); \
-    } else {                                                                               \
-      hipLaunchKernelGGL((k_apply<R, TK, TJ, false>), grid, block, 0, s, op, in, out, chunk, ep); \
-    }                                                                                      \
-    return hipGetLastError();                                                              \
-  }
+  if (t.tk == TK && t.tj == TJ)                                                            \
+    return launch_march<R, TK, TJ>(grid, block, s, op, in, out, chunk, ep, t.nt != 0);
   APHID_TILE(16, 16)
   APHID_TILE(32, 8)
   APHID_TILE(64, 4)
@@ -225,6 +234,13 @@ struct aphid_ctx {
   c128* invd = nullptr;
   c64* invdf = nullptr;
   bool have_k2 = false, have_diag = false;
+  // compact k^2 of the complex128 stencil (OpArgs::k2im / k2re): 3*nn
+  // imaginary parts, then the tables rx (ny*nz), ry (nx*nz), rz (nx*ny);
+  // used when k2c_ok (checked against the grids) and k2c_on (the caller's
+  // switch, aphid_set_compact_k2)
+  double* k2c = nullptr;
+  int* k2c_flag = nullptr;
+  bool k2c_ok = false, k2c_on = true;
   // Krylov vectors, complex128 and complex64 sets
   c128* V[9] = {};  // x b v p t d s r spare
   bool vborrowed = false;  // V[] are slices of a batch's flat vectors (not owned)
@@ -278,7 +294,7 @@ void free_mesh(aphid_ctx* c) {
     c->giters[p] = 0;
   }
   void* ptrs[] = {c->tab64, c->tab32, c->dmet, c->sig, c->k2, c->k2f, c->diag, c->invd, c->invdf,
-                  c->stage, c->stagef, c->hist};
+                  c->stage, c->stagef, c->hist, c->k2c, c->k2c_flag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& v : c->V) {
@@ -291,6 +307,9 @@ void free_mesh(aphid_ctx* c) {
   c->tab64 = nullptr; c->tab32 = nullptr; c->dmet = nullptr; c->sig = nullptr; c->k2 = nullptr;
   c->k2f = nullptr; c->diag = nullptr; c->invd = nullptr; c->invdf = nullptr; c->stage = nullptr;
   c->stagef = nullptr; c->hist = nullptr; c->hist_cap = 0;
+  c->k2c = nullptr; c->k2c_flag = nullptr; c->k2c_ok = false;
+  c->op64.k2im = nullptr;
+  for (auto& t : c->op64.k2re) t = nullptr;
   if (c->seg) (void)hipFree(c->seg);
   c->seg = nullptr;
   c->seg_n = 0;
@@ -354,7 +373,49 @@ int ensure_vectors(aphid_ctx* c, bool f32) {
   return 0;
 }
 
-// derived complex64 operator data (k2, 1/diag)
+// Points the complex128 operator at the compact k^2 when it is valid and
+// switched on; a captured iteration graph holds the old kernel arguments,
+// so it is dropped when the choice changes.
+void select_k2c(aphid_ctx* c) {
+  const bool use = c->k2c_ok && c->k2c_on;
+  if (use == (c->op64.k2im != nullptr)) return;
+  if (c->gexec[0]) (void)hipGraphExecDestroy(c->gexec[0]), c->gexec[0] = nullptr;
+  if (use) {
+    c->op64.k2im = c->k2c;
+    c->op64.k2re[0] = c->k2c + 3 * c->nn;
+    c->op64.k2re[1] = c->op64.k2re[0] + (size_t)c->ny * c->nz;
+    c->op64.k2re[2] = c->op64.k2re[1] + (size_t)c->nx * c->nz;
+  } else {
+    c->op64.k2im = nullptr;
+    for (auto& t : c->op64.k2re) t = nullptr;
+  }
+}
+
+// the compact k^2 from the current grids, and whether it holds them exactly
+int build_k2c(aphid_ctx* c) {
+  const size_t n = 3 * (size_t)c->nn + (size_t)c->ny * c->nz + (size_t)c->nx * c->nz +
+                   (size_t)c->nx * c->ny;
+  if (!c->k2c) CK(hipMalloc(&c->k2c, sizeof(double) * n));
+  if (!c->k2c_flag) CK(hipMalloc(&c->k2c_flag, sizeof(int)));
+  double* rx = c->k2c + 3 * c->nn;
+  double* ry = rx + (size_t)c->ny * c->nz;
+  double* rz = ry + (size_t)c->nx * c->nz;
+  CK(hipMemsetAsync(c->k2c_flag, 0, sizeof(int), c->s));
+  hipLaunchKernelGGL(k_k2c_extract, dim3(grid_for(c->nn)), dim3(kBlock), 0, c->s, c->k2, c->k2c,
+                     rx, ry, rz, dims(c));
+  hipLaunchKernelGGL(k_k2c_check, dim3(grid_for(c->nn)), dim3(kBlock), 0, c->s, c->k2, rx, ry, rz,
+                     dims(c), c->k2c_flag);
+  CK(hipGetLastError());
+  int bad = 1;
+  CK(hipMemcpyAsync(&bad, c->k2c_flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  CK(hipStreamSynchronize(c->s));
+  c->k2c_ok = bad == 0;
+  select_k2c(c);
+  return 0;
+}
+
+// derived operator data: complex64 copies (k2, 1/diag) and the compact
+// complex128 k^2
 int refresh_f32(aphid_ctx* c) {
   const long long n4 = 4 * c->nn;
   hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n4)), dim3(kBlock), 0, c->s, c->k2,
@@ -362,7 +423,7 @@ int refresh_f32(aphid_ctx* c) {
   hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n4)), dim3(kBlock), 0, c->s,
                      c->invd, c->invdf, n4);
   CK(hipGetLastError());
-  return 0;
+  return build_k2c(c);
 }
 
 int build_diag(aphid_ctx* c) {
@@ -2195,6 +2256,14 @@ int aphid_set_reproducible(aphid_ctx* c, int on) {
   return 0;
 }
 
+int aphid_set_compact_k2(aphid_ctx* c, int on, int* active) {
+  if (!c) return fail(APHID_E_ARG, "NULL context");
+  c->k2c_on = on != 0;
+  if (c->op64.k2 && c->k2c) select_k2c(c);
+  if (active) *active = c->op64.k2im != nullptr ? 1 : 0;
+  return 0;
+}
+
 int aphid_time_traffic(aphid_ctx* c, int reps, double* seconds) {
   if (!c || !seconds || reps < 1) return fail(APHID_E_ARG, "bad argument");
   if (!c->have_k2) return fail(APHID_E_STATE, "operator not set up");
@@ -2209,14 +2278,18 @@ int aphid_time_traffic(aphid_ctx* c, int reps, double* seconds) {
   const int ntk = (op.nz + t.tk - 1) / t.tk, ntj = (op.ny + t.tj - 1) / t.tj;
   const dim3 grid((unsigned)ntk * ntj * ((op.nx + t.chunk - 1) / t.chunk)), block(t.tk * t.tj);
   auto launch = [&]() -> hipError_t {
+    const bool ck = op.k2im != nullptr;  // the compact k^2 loads of the stencil
+#define APHID_TRAFFIC(TK, TJ, NT, CKV)                                                        \
+  hipLaunchKernelGGL((k_apply_traffic<double, TK, TJ, NT, CKV>), grid, block, 0, c->s, op,     \
+                     c->V[3], c->V[8], t.chunk)
 #define APHID_TILE(TK, TJ)                                                                  \
     if (t.tk == TK && t.tj == TJ) {                                                         \
-      if (t.nt)                                                                             \
-        hipLaunchKernelGGL((k_apply_traffic<double, TK, TJ, true>), grid, block, 0, c->s,   \
-                           op, c->V[3], c->V[8], t.chunk);                                  \
-      else                                                                                  \
-        hipLaunchKernelGGL((k_apply_traffic<double, TK, TJ, false>), grid, block, 0, c->s,  \
-                           op, c->V[3], c->V[8], t.chunk);                                  \
+      if (ck) {                                                                             \
+        if (t.nt) APHID_TRAFFIC(TK, TJ, true, true); else APHID_TRAFFIC(TK, TJ, false, true); \
+      } else {                                                                              \
+        if (t.nt) APHID_TRAFFIC(TK, TJ, true, false);                                       \
+        else APHID_TRAFFIC(TK, TJ, false, false);                                           \
+      }                                                                                     \
       return hipGetLastError();                                                             \
     }
     APHID_TILE(16, 16)
@@ -2228,6 +2301,7 @@ int aphid_time_traffic(aphid_ctx* c, int reps, double* seconds) {
     APHID_TILE(64, 8)
     APHID_TILE(16, 32)
 #undef APHID_TILE
+#undef APHID_TRAFFIC
     return hipErrorInvalidValue;
   };
   CK(launch());  // warm-up

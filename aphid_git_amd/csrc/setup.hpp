@@ -181,6 +181,49 @@ __global__ void k_k2(const c128* __restrict__ sig, const double* __restrict__ Dx
   }
 }
 
+// Compact k^2 for the stencil (OpArgs::k2im / k2re): the imaginary parts
+// of k2x, k2y, k2z, and the real parts of the i = 0, j = 0 and k = 0 slices
+// as 2-D tables rx(j,k), ry(i,k), rz(i,j).  For a sigma-hat whose imaginary
+// part is the same in every cell (model R), each edge grid's real part is
+// that value times metric weights of the two transverse axes
+// (gen_k2_{x,y,z}grid, src/aphid.f90:1306-1440), so the tables hold every
+// real part; k_k2c_check verifies this entry by entry.
+__global__ void k_k2c_extract(const c128* __restrict__ k2, double* __restrict__ im,
+                              double* __restrict__ rx, double* __restrict__ ry,
+                              double* __restrict__ rz, Dims d) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < d.nn;
+       q += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    d.ijk(q, i, j, k);
+    for (int c = 0; c < 3; ++c) im[c * d.nn + q] = k2[c * d.nn + q].im;
+    if (i == 0) rx[j * d.nz + k] = k2[q].re;
+    if (j == 0) ry[i * d.nz + k] = k2[d.nn + q].re;
+    if (k == 0) rz[i * d.ny + j] = k2[2 * d.nn + q].re;
+  }
+}
+
+// Sets *bad = 1 if any real part of an edge grid (padding excluded) differs
+// from its table entry in any bit.
+__global__ void k_k2c_check(const c128* __restrict__ k2, const double* __restrict__ rx,
+                            const double* __restrict__ ry, const double* __restrict__ rz, Dims d,
+                            int* bad) {
+  bool diff = false;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < d.nn;
+       q += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    d.ijk(q, i, j, k);
+    if (i < d.nx - 1 && __double_as_longlong(k2[q].re) != __double_as_longlong(rx[j * d.nz + k]))
+      diff = true;
+    if (j < d.ny - 1 &&
+        __double_as_longlong(k2[d.nn + q].re) != __double_as_longlong(ry[i * d.nz + k]))
+      diff = true;
+    if (k < d.nz - 1 &&
+        __double_as_longlong(k2[2 * d.nn + q].re) != __double_as_longlong(rz[i * d.ny + j]))
+      diff = true;
+  }
+  if (diff) *bad = 1;  // every writer stores the same value
+}
+
 // Jacobi diagonal (gen_diag, src/aphid.f90:1542-1725), including its
 // boundary quirks: transverse centred spacings recomputed as
 // 0.5*(d(j)+d(j-1)) with d(0)=d(1), d(n)=d(n-1); diag_p = 1 on boundary

@@ -42,6 +42,19 @@ struct OpArgs {
   const R* rdE[3];
   const R* dN[3];
   const R* rdN[3];
+  // Compact k^2 (complex128 x-march; nullptr: off).  When every cell's
+  // sigma-hat has the same imaginary part (model R: sigma + i omega eps0),
+  // the real parts of the edge grids depend on two coordinates only: the
+  // transverse metric weights of gen_k2_{x,y,z}grid (src/aphid.f90:1306-1440)
+  // times that one value.  The stencil then reads the imaginary parts of
+  // k2x, k2y, k2z (k2im, 3*nn values, 8 bytes each) and their real parts
+  // from the 2-D tables k2re[0](j,k) = [j*nz + k], k2re[1](i,k) = [i*nz + k],
+  // k2re[2](i,j) = [i*ny + j]; k2p stays whole.  The tables are taken from
+  // the stored grids and checked against every stored real part bit for bit
+  // before they are used (k_k2c_extract, k_k2c_check), so the stencil sees
+  // the same complex values either way.
+  const R* k2im = nullptr;
+  const R* k2re[3] = {nullptr, nullptr, nullptr};
 };
 
 // Optional epilogue of the apply: eps = in^T out (bilinear, FP64) reduced
@@ -272,8 +285,8 @@ struct MarchAcc {
   }
 };
 
-template <typename R, int TK, int TJ, bool NTS, bool SEG = false>
-__global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* __restrict__ in,
+template <typename R, int TK, int TJ, bool NTS, bool SEG = false, bool CK = false>
+__global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, const cplx<R>* __restrict__ in,
                                                   cplx<R>* __restrict__ out, int chunk,
                                                   ApplyEpilogue ep) {
   using C = cplx<R>;
@@ -352,6 +365,41 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
   const int hl = (hj + 1) * LK + (hk + 1);
   const int ol = (tj + 1) * LK + (tk + 1);
 
+  // k^2 grid c (0..2) of the own column at plane i: whole, or (CK) the
+  // imaginary part and the real part from its 2-D table (0 outside the
+  // domain).  In the edge grids' padding (i = nx-1 for k2x, j = ny-1 for
+  // k2y, k = nz-1 for k2z) the table gives a nonzero real part where the
+  // whole grid holds 0; rows4 never reads those entries.  The ry and rz
+  // tables are contiguous (k2re[2] = k2re[1] + nx*nz), so both are read
+  // through one base with 32-bit byte offsets.
+  constexpr unsigned RS = sizeof(R);
+  const unsigned rzb = (unsigned)nx * nz * RS;
+  const R rex = (CK && own) ? ldo(op.k2re[0], ((unsigned)j * nz + k) * RS) : R(0);
+  auto ldk = [&](int c, int i) -> C {
+    if constexpr (!CK) {
+      return ldc(op.k2, c, i);
+    } else {
+      if (!(own && i >= 0 && i < nx)) return C{R(0), R(0)};
+      const R im = ldo(op.k2im, (c * nnb + (unsigned)i * plb + colb) / 2u);
+      const R re = c == 0 ? rex
+                          : ldo(op.k2re[1], c == 1 ? ((unsigned)i * nz + k) * RS
+                                                   : rzb + ((unsigned)i * ny + j) * RS);
+      return C{re, im};
+    }
+  };
+  // the k^2 value a halo cell stages (grid hc of the cell at (gj, gk))
+  auto ldh = [&](unsigned g, int i) -> C {
+    if constexpr (!CK) {
+      return ldo(op.k2, hc * nnb + g);
+    } else {
+      if (hc == 3) return ldo(op.k2, 3 * nnb + g);
+      const R im = ldo(op.k2im, (hc * nnb + g) / 2u);
+      const R re = ldo(op.k2re[1], hc == 1 ? ((unsigned)i * nz + gk) * RS
+                                           : rzb + ((unsigned)i * ny + gj) * RS);
+      return C{re, im};
+    }
+  };
+
   C pa[4], ca[4], na[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -359,20 +407,32 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     ca[c] = ldc(in, c, i0);
   }
   const C* __restrict__ K2 = op.k2;
-  C k2x_p = ldc(K2, 0, i0 - 1), k2x_c = ldc(K2, 0, i0), k2p_c = ldc(K2, 3, i0);
+  // the x-neighbour queue of k2x: whole values, or (CK) imaginary parts
+  // only, all sharing the column's table value rex
+  using KX = std::conditional_t<CK, R, C>;
+  auto ldkx = [&](int i) -> KX {
+    if constexpr (CK) return (own && i >= 0 && i < nx) ? ldo(op.k2im, (unsigned)i * (plb / 2u) + colb / 2u) : R(0);
+    else return ldk(0, i);
+  };
+  auto kxv = [&](KX v) -> C {
+    if constexpr (CK) return C{rex, v};
+    else return v;
+  };
+  KX k2x_p = ldkx(i0 - 1), k2x_c = ldkx(i0);
+  C k2p_c = ldc(K2, 3, i0);
 
   double eacc_re = 0.0, eacc_im = 0.0;
 
   for (int i = i0; i < i1; ++i) {
     // stage plane i: the loads this step needs (own k2y, k2z and the
     // halo) go out first, since waits on loads complete in issue order
-    sm[4][ol] = ldc(K2, 1, i);
-    sm[5][ol] = ldc(K2, 2, i);
+    sm[4][ol] = ldk(1, i);
+    sm[5][ol] = ldk(2, i);
     if (hv) {
       const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
       for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : C{R(0), R(0)};
-      sm[3 + hc][hl] = hin ? ldo(K2, hc * nnb + g) : C{R(0), R(0)};
+      sm[3 + hc][hl] = hin ? ldh(g, i) : C{R(0), R(0)};
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
@@ -380,7 +440,8 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
     // plane i+1 of the own column (the x-neighbours of the next step)
 #pragma unroll
     for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
-    const C k2x_n = ldc(K2, 0, i + 1), k2p_n = ldc(K2, 3, i + 1);
+    const KX k2x_n = ldkx(i + 1);
+    const C k2p_n = ldc(K2, 3, i + 1);
     __syncthreads();
 
     if (own) {
@@ -390,7 +451,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply(OpArgs<R> op, const cplx<R>* _
                         tyj[4][tj], tyj[5][tj], tyj[6][tj], tyj[7][tj]};
       const Ax1<R> Z = {tzk[0][tk], tzk[1][tk], tzk[2][tk], tzk[3][tk],
                         tzk[4][tk], tzk[5][tk], tzk[6][tk], tzk[7][tk]};
-      MarchAcc<R, TK, TJ> a{sm, pa, ca, na, ol, k2x_p, k2x_c, k2p_c, k2p_n};
+      MarchAcc<R, TK, TJ> a{sm, pa, ca, na, ol, kxv(k2x_p), kxv(k2x_c), k2p_c, k2p_n};
       C rx, ry, rz, rp;
       rows4<R>(i, j, k, nx, ny, nz, X, Y, Z, a, rx, ry, rz, rp);
       sto_h<NTS>(out, ob, rx);
@@ -454,7 +515,7 @@ __global__ void k_seg_reduce(const double* __restrict__ seg, long long n, long l
 // k_apply<R,TK,TJ,NTS>, so the same number of blocks share a CU.  Its time is the floor the march's access pattern sets; the loaded
 // values are folded into the (meaningless) outputs so that no load is
 // dropped.
-template <typename R, int TK, int TJ, bool NTS>
+template <typename R, int TK, int TJ, bool NTS, bool CK = false>
 __global__ __launch_bounds__(TK* TJ) void k_apply_traffic(OpArgs<R> op,
                                                           const cplx<R>* __restrict__ in,
                                                           cplx<R>* __restrict__ out, int chunk) {
@@ -494,27 +555,49 @@ __global__ __launch_bounds__(TK* TJ) void k_apply_traffic(OpArgs<R> op,
     return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb)
                                      : C{R(0), R(0)};
   };
+  // k^2 grids 0..2 as k_apply<..., CK> reads them (imaginary part + table)
+  auto ldk = [&](int c, int i) -> C {
+    if constexpr (!CK) {
+      return ldc(K2, c, i);
+    } else {
+      if (!(own && i >= 0 && i < nx)) return C{R(0), R(0)};
+      const R im = ldo(op.k2im, (c * nnb + (unsigned)i * plb + colb) / 2u);
+      const R re = c == 0 ? op.k2re[0][j * nz + k]
+                          : (c == 1 ? op.k2re[1][i * nz + k] : op.k2re[2][i * ny + j]);
+      return C{re, im};
+    }
+  };
+  auto ldh = [&](unsigned g, int i) -> C {
+    if constexpr (!CK) {
+      return ldo(K2, hc * nnb + g);
+    } else {
+      if (hc == 3) return ldo(K2, 3 * nnb + g);
+      const R im = ldo(op.k2im, (hc * nnb + g) / 2u);
+      const R re = hc == 1 ? op.k2re[1][i * nz + gk] : op.k2re[2][i * ny + gj];
+      return C{re, im};
+    }
+  };
   hold[threadIdx.x] = C{R(0), R(0)};
-  C acc = ldc(K2, 0, i0 - 1);
+  C acc = ldk(0, i0 - 1);
   C ca[4], na[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     acc += ldc(in, c, i0 - 1);
     ca[c] = ldc(in, c, i0);
   }
-  C kx = ldc(K2, 0, i0), kp = ldc(K2, 3, i0);
+  C kx = ldk(0, i0), kp = ldc(K2, 3, i0);
   for (int i = i0; i < i1; ++i) {
-    const C ky = ldc(K2, 1, i), kz = ldc(K2, 2, i);
+    const C ky = ldk(1, i), kz = ldk(2, i);
     C h = {R(0), R(0)};
     if (hin) {
       const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
       for (int c = 0; c < 4; ++c) h += ldo(in, c * nnb + g);
-      h += ldo(K2, hc * nnb + g);
+      h += ldh(g, i);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
-    const C kxn = ldc(K2, 0, i + 1), kpn = ldc(K2, 3, i + 1);
+    const C kxn = ldk(0, i + 1), kpn = ldc(K2, 3, i + 1);
     if (own) {
       const unsigned ob = (unsigned)i * plb + colb;
       const C s = acc + h + kx + kp + ky + kz;

@@ -362,6 +362,15 @@ class Solver:
         launch shape (eps from fixed-order segment partials)."""
         _check(lib().aphid_set_reproducible(self._ctx, int(bool(on))), "aphid_set_reproducible")
 
+    def set_compact_k2(self, on: bool = True) -> bool:
+        """Compact complex128 k^2 in the stencil (imaginary parts plus 2-D
+        tables of the real parts, used only where exact); returns whether it
+        is in use."""
+        a = C.c_int()
+        _check(lib().aphid_set_compact_k2(self._ctx, int(bool(on)), C.byref(a)),
+               "aphid_set_compact_k2")
+        return bool(a.value)
+
     def time_traffic(self, reps: int = 20) -> float:
         """Seconds per launch of the tuned stencil tile's loads and stores
         alone (no arithmetic): the floor its access pattern sets."""
@@ -585,6 +594,7 @@ EXPORTS = ["aphid_ctx_create", "aphid_ctx_destroy", "aphid_last_error", "aphid_s
            "aphid_get_diag", "aphid_apply", "aphid_apply_noalloc", "aphid_apply_prec", "aphid_build_rhs", "aphid_qmr_solve", "aphid_get_tile",
            "aphid_set_tile", "aphid_time_apply", "aphid_load_rhs", "aphid_solve_resident",
            "aphid_fetch_solution", "aphid_time_iteration", "aphid_time_copy", "aphid_time_traffic", "aphid_set_reproducible",
+           "aphid_set_compact_k2",
            "aphid_batch_create", "aphid_batch_destroy", "aphid_batch_add_operator",
            "aphid_batch_add_system", "aphid_batch_solve", "aphid_batch_get_solution",
            "aphid_batch_set_guess", "aphid_batch_set_sweep_guess", "aphid_batch_set_history",

@@ -209,6 +209,22 @@ int aphid_time_apply(aphid_ctx *ctx, int precision, int reps, double *seconds);
  * partials.  Default off. */
 int aphid_set_reproducible(aphid_ctx *ctx, int on);
 
+/* Compact k^2 for the complex128 x-march stencil (no reference counterpart:
+ * a storage choice inside the operator apply, inprod_{x,y,z,p} of
+ * src/aphid.f90:803-1282, that leaves every value it computes unchanged).
+ * When every cell's sigma-hat has the same imaginary part (model R:
+ * sigma + i omega eps0, src/aphid.f90:379-449), the real parts of the edge
+ * grids k2x, k2y, k2z depend on two coordinates only; the stencil then
+ * reads their imaginary parts (8 bytes instead of 16) and takes the real
+ * parts from 2-D tables.  The library checks every real part against the
+ * tables, bit for bit, each time k^2 is set, and uses the compact form only
+ * if all agree (so models C, D and I, or k^2 given by aphid_set_k2 without
+ * that structure, keep the whole grids).  on = 0 switches it off (the
+ * stencil then reads the whole grids; the results are the same bits).
+ * *active (may be NULL) returns whether the compact form is in use.
+ * Default on. */
+int aphid_set_compact_k2(aphid_ctx *ctx, int on, int *active);
+
 /* Mean seconds per launch of the stencil's memory traffic alone: the tuned
  * complex128 march tile's loads (own columns of the four fields and four
  * k^2 grids, halo cells) and stores (with its store hint) with no
