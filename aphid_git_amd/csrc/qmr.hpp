@@ -249,8 +249,10 @@ __global__ void k_vtnorm(const vec<V>* __restrict__ t, const vec<V>* __restrict_
   block_sum<1>(a);
   block_partial<1>(a, part);
 }
-// step-4 scalars from rho' = ||t - beta v|| (src/aphid_qmr.f90:342-345)
-__device__ __forceinline__ void vt_scalars(QmrState* st, double rho) {
+// step-4 scalars from rho' = ||t - beta v|| (src/aphid_qmr.f90:342-345),
+// on the state or on a register copy of the fields they use (VtState)
+template <typename S>
+__device__ __forceinline__ void vt_scalars(S* st, double rho) {
   const c128 beta = st->beta;
   st->rho_np = rho;
   const double bb = beta.re * beta.re + beta.im * beta.im;
@@ -264,12 +266,38 @@ __device__ __forceinline__ void vt_scalars(QmrState* st, double rho) {
     st->done = 1;
   }
 }
+struct VtState {
+  c128 beta, eta;
+  double rho_np, rho_n, theta_n, theta_nm, c_n, c_nm, k2s;
+  int status, done;
+};
 
+// The per-iteration finalize kernels copy the state at their start, so that
+// its load is in flight together with the partials' loads (one memory round
+// trip instead of one for the done flag, one for the partials and one for
+// the scalars), update the copy and store it back; after the stop they
+// leave the state alone.
 __global__ void k_fin_vtnorm(const double* part, int nb, QmrState* st) {
-  if (st->done) return;
+  VtState s;
+  s.done = st->done;
+  s.beta = st->beta;
+  s.eta = st->eta;
+  s.rho_n = st->rho_n;
+  s.theta_nm = st->theta_nm;
+  s.c_nm = st->c_nm;
+  s.status = st->status;
   double tot[1];
   finalize_sum<1>(part, nb, tot);
-  if (threadIdx.x == 0) vt_scalars(st, sqrt(tot[0]));
+  if (threadIdx.x == 0 && !s.done) {
+    vt_scalars(&s, sqrt(tot[0]));
+    st->rho_np = s.rho_np;
+    st->theta_n = s.theta_n;
+    st->c_n = s.c_n;
+    st->eta = s.eta;
+    st->k2s = s.k2s;
+    st->status = s.status;
+    st->done = s.done;
+  }
 }
 
 // one element of step D: s, r, v updates; accumulates ||r||^2 and
@@ -421,10 +449,14 @@ __device__ __forceinline__ void update_scalars(QmrState* st, const double (&tot)
 }
 
 __global__ void k_fin_update(const double* part, int nb, QmrState* st, double* hist) {
-  if (st->done) return;
+  QmrState s;
+  if (threadIdx.x == 0) s = *st;
   double tot[3];
   finalize_sum<3>(part, nb, tot);
-  if (threadIdx.x == 0) update_scalars(st, tot, hist);
+  if (threadIdx.x == 0 && !s.done) {
+    update_scalars(&s, tot, hist);
+    *st = s;
+  }
 }
 
 // ||b - t||^2 / ||b||^2 for the true residual (t = K x)

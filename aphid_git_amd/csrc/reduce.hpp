@@ -55,13 +55,29 @@ __device__ __forceinline__ void block_partial(const double (&v)[NV], double* par
 
 // In a one-block finalize kernel: the fixed-order total of nb partials
 // (thread t sums blocks t, t+T, ...; then a block tree), valid in thread 0.
+// The loads of up to 8 partials per thread are issued together before any
+// is added (one memory round trip for 2048 partials at 256 threads instead
+// of one per partial); the additions keep the order b = t, t+T, t+2T, ...
 template <int NV>
 __device__ __forceinline__ void finalize_sum(const double* __restrict__ part, int nb, double (&tot)[NV]) {
+  constexpr int U = 8;
 #pragma unroll
   for (int q = 0; q < NV; ++q) tot[q] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+  const int T = blockDim.x;
+  for (int b0 = threadIdx.x; b0 < nb; b0 += U * T) {
+    double x[U][NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] += part[(size_t)b * NV + q];
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * T;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) x[u][q] = b < nb ? part[(size_t)b * NV + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (b0 + u * T < nb)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] += x[u][q];
+  }
   block_sum<NV>(tot);
 }
 

@@ -695,13 +695,17 @@ __global__ __launch_bounds__(TK* TJ) void k_apply_direct(OpArgs<R> op, const cpl
 
 // eps = p^T K p (the partials of k_apply), beta = eps/del (src/aphid_qmr.f90:329-330)
 __global__ void k_fin_eps(const double* part, int nb, QmrState* st) {
-  if (st->done) return;
+  // the state's fields are loaded with the partials (one round trip)
+  const int done = st->done;
+  const c128 del = st->del;
   double tot[2];
   finalize_sum<2>(part, nb, tot);
-  if (threadIdx.x == 0) {
-    st->eps = {tot[0], tot[1]};
-    st->beta = cdiv(st->eps, st->del);
-    if (is_zero(st->eps) || !finite_c(st->beta)) {
+  if (threadIdx.x == 0 && !done) {
+    const c128 eps = {tot[0], tot[1]};
+    const c128 beta = cdiv(eps, del);
+    st->eps = eps;
+    st->beta = beta;
+    if (is_zero(eps) || !finite_c(beta)) {
       st->status = ST_BREAK;
       st->done = 1;
     }
