@@ -554,6 +554,7 @@ def main():
         t_traffic = s.time_traffic(20)
         kt = s.time_iteration(0, 10)
         tile = s.get_tile(0)
+        k2_compact = s.set_compact_k2(True)  # the default; reports whether it is in use
         s_bytes = 16
         B_mv = 3 * M * s_bytes
         per_kernel = {
@@ -610,7 +611,15 @@ def main():
                                            "(own columns of 4 fields and 4 k^2 grids, halo "
                                            "cells) and 4 stores per node, no LDS, barriers "
                                            "or arithmetic, same occupancy"},
-                         "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]}},
+                         "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]},
+                         "k2_compact": k2_compact,
+                         "k2_compact_note": "with the compact k^2 (model R) the stencil reads "
+                                            "the edge grids k2x, k2y, k2z as 8-byte imaginary "
+                                            "parts plus 2-D tables of their real parts (the "
+                                            "same values, checked bit for bit), so it moves "
+                                            "2.625*M*16 of the 3*M*16 algorithmic bytes; "
+                                            "achieved and frac count the algorithmic bytes, "
+                                            "traffic is what the counters measured"},
             "roofline_dominant_kernel": {"kernel": dom, **kernels[dom], "peak": HBM_PEAK_GBS},
             "kernels": kernels,
             "iteration": {"seconds": kt[4], "algorithmic_bytes": it_bytes,
