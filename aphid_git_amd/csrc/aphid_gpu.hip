@@ -63,7 +63,7 @@ const Tile kTiles[] = {{16, 16, 0}, {32, 8, 0}, {64, 4, 0}, {16, 8, 0}, {32, 4, 
                        {32, 16, 0}, {64, 8, 0}, {16, 32, 0}};
 
 // the x-march stencil of one tile: store hint (nt), reproducible eps
-// partials (ep.seg) and, for complex128, the compact k^2 (op.k2im; not with
+// partials (ep.seg) and, for complex128, the compact k^2 (op.k2rec; not with
 // the segment partials, whose registers it would spill)
 template <typename R, int TK, int TJ>
 hipError_t launch_march(dim3 grid, dim3 block, hipStream_t s, const OpArgs<R>& op,
@@ -71,7 +71,7 @@ hipError_t launch_march(dim3 grid, dim3 block, hipStream_t s, const OpArgs<R>& o
 #define APHID_MARCH(NT, SEG, CKV)                                                            \
   hipLaunchKernelGGL((k_apply<R, TK, TJ, NT, SEG, CKV>), grid, block, 0, s, op, in, out, chunk, ep)
   if constexpr (std::is_same<R, double>::value) {
-    const bool ck = op.k2im != nullptr;
+    const bool ck = op.k2rec != nullptr;
     if (ep.seg) {
       if (nt) APHID_MARCH(true, true, false); else APHID_MARCH(false, true, false);
     } else if (ck) {
@@ -234,13 +234,15 @@ struct aphid_ctx {
   c128* invd = nullptr;
   c64* invdf = nullptr;
   bool have_k2 = false, have_diag = false;
-  // compact k^2 of the complex128 stencil (OpArgs::k2im / k2re): 3*nn
-  // imaginary parts, then the tables rx (ny*nz), ry (nx*nz), rz (nx*ny);
-  // used when k2c_ok (checked against the grids) and k2c_on (the caller's
-  // switch, aphid_set_compact_k2)
+  // compact k^2 of the complex128 stencil (OpArgs::k2rec / k2re / k2pc):
+  // 2*nn pairs of doubles, the tables rx (ny*nz), ry (nx*nz), rz (nx*ny),
+  // then nn k2p codes (bytes) from the bit pattern k2c_base; used when
+  // k2c_ok (checked against the grids) and k2c_on (the caller's switch,
+  // aphid_set_compact_k2)
   double* k2c = nullptr;
   int* k2c_flag = nullptr;
   bool k2c_ok = false, k2c_on = true;
+  unsigned long long k2c_base = 0;
   // Krylov vectors, complex128 and complex64 sets
   c128* V[9] = {};  // x b v p t d s r spare
   bool vborrowed = false;  // V[] are slices of a batch's flat vectors (not owned)
@@ -308,8 +310,10 @@ void free_mesh(aphid_ctx* c) {
   c->k2f = nullptr; c->diag = nullptr; c->invd = nullptr; c->invdf = nullptr; c->stage = nullptr;
   c->stagef = nullptr; c->hist = nullptr; c->hist_cap = 0;
   c->k2c = nullptr; c->k2c_flag = nullptr; c->k2c_ok = false;
-  c->op64.k2im = nullptr;
+  c->op64.k2rec = nullptr;
   for (auto& t : c->op64.k2re) t = nullptr;
+  c->op64.k2pc = nullptr;
+  c->op64.k2p_base = 0;
   if (c->seg) (void)hipFree(c->seg);
   c->seg = nullptr;
   c->seg_n = 0;
@@ -378,33 +382,49 @@ int ensure_vectors(aphid_ctx* c, bool f32) {
 // so it is dropped when the choice changes.
 void select_k2c(aphid_ctx* c) {
   const bool use = c->k2c_ok && c->k2c_on;
-  if (use == (c->op64.k2im != nullptr)) return;
+  const bool same = use ? c->op64.k2rec != nullptr && c->op64.k2p_base == c->k2c_base
+                        : c->op64.k2rec == nullptr;
+  if (same) return;
   if (c->gexec[0]) (void)hipGraphExecDestroy(c->gexec[0]), c->gexec[0] = nullptr;
   if (use) {
-    c->op64.k2im = c->k2c;
-    c->op64.k2re[0] = c->k2c + 3 * c->nn;
+    c->op64.k2rec = c->k2c;
+    c->op64.k2yz_off = (unsigned)(2 * c->nn * sizeof(double));
+    c->op64.k2re[0] = c->k2c + 4 * c->nn;
     c->op64.k2re[1] = c->op64.k2re[0] + (size_t)c->ny * c->nz;
     c->op64.k2re[2] = c->op64.k2re[1] + (size_t)c->nx * c->nz;
+    c->op64.k2pc = reinterpret_cast<const unsigned char*>(c->op64.k2re[2] + (size_t)c->nx * c->ny);
+    c->op64.k2p_base = c->k2c_base;
   } else {
-    c->op64.k2im = nullptr;
+    c->op64.k2rec = nullptr;
     for (auto& t : c->op64.k2re) t = nullptr;
+    c->op64.k2pc = nullptr;
+    c->op64.k2p_base = 0;
   }
 }
 
 // the compact k^2 from the current grids, and whether it holds them exactly
+// (the codes' base: 128 below the bit pattern of k2p's real part at the
+// first node)
 int build_k2c(aphid_ctx* c) {
-  const size_t n = 3 * (size_t)c->nn + (size_t)c->ny * c->nz + (size_t)c->nx * c->nz +
-                   (size_t)c->nx * c->ny;
-  if (!c->k2c) CK(hipMalloc(&c->k2c, sizeof(double) * n));
+  const size_t nd = 4 * (size_t)c->nn + (size_t)c->ny * c->nz + (size_t)c->nx * c->nz +
+                    (size_t)c->nx * c->ny;
+  if (!c->k2c) CK(hipMalloc(&c->k2c, sizeof(double) * nd + (size_t)c->nn));
   if (!c->k2c_flag) CK(hipMalloc(&c->k2c_flag, sizeof(int)));
-  double* rx = c->k2c + 3 * c->nn;
+  double* rx = c->k2c + 4 * c->nn;
   double* ry = rx + (size_t)c->ny * c->nz;
   double* rz = ry + (size_t)c->nx * c->nz;
+  unsigned char* pc = reinterpret_cast<unsigned char*>(c->k2c + nd);
+  double p0 = 0.0;
+  CK(hipMemcpyAsync(&p0, &c->k2[3 * c->nn].re, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  CK(hipStreamSynchronize(c->s));
+  unsigned long long b0;
+  std::memcpy(&b0, &p0, sizeof(b0));
+  c->k2c_base = b0 - 128ull;
   CK(hipMemsetAsync(c->k2c_flag, 0, sizeof(int), c->s));
   hipLaunchKernelGGL(k_k2c_extract, dim3(grid_for(c->nn)), dim3(kBlock), 0, c->s, c->k2, c->k2c,
-                     rx, ry, rz, dims(c));
+                     c->k2c + 2 * c->nn, rx, ry, rz, pc, c->k2c_base, dims(c));
   hipLaunchKernelGGL(k_k2c_check, dim3(grid_for(c->nn)), dim3(kBlock), 0, c->s, c->k2, rx, ry, rz,
-                     dims(c), c->k2c_flag);
+                     c->k2c_base, dims(c), c->k2c_flag);
   CK(hipGetLastError());
   int bad = 1;
   CK(hipMemcpyAsync(&bad, c->k2c_flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -2260,7 +2280,7 @@ int aphid_set_compact_k2(aphid_ctx* c, int on, int* active) {
   if (!c) return fail(APHID_E_ARG, "NULL context");
   c->k2c_on = on != 0;
   if (c->op64.k2 && c->k2c) select_k2c(c);
-  if (active) *active = c->op64.k2im != nullptr ? 1 : 0;
+  if (active) *active = c->op64.k2rec != nullptr ? 1 : 0;
   return 0;
 }
 
@@ -2278,7 +2298,7 @@ int aphid_time_traffic(aphid_ctx* c, int reps, double* seconds) {
   const int ntk = (op.nz + t.tk - 1) / t.tk, ntj = (op.ny + t.tj - 1) / t.tj;
   const dim3 grid((unsigned)ntk * ntj * ((op.nx + t.chunk - 1) / t.chunk)), block(t.tk * t.tj);
   auto launch = [&]() -> hipError_t {
-    const bool ck = op.k2im != nullptr;  // the compact k^2 loads of the stencil
+    const bool ck = op.k2rec != nullptr;  // the compact k^2 loads of the stencil
 #define APHID_TRAFFIC(TK, TJ, NT, CKV)                                                        \
   hipLaunchKernelGGL((k_apply_traffic<double, TK, TJ, NT, CKV>), grid, block, 0, c->s, op,     \
                      c->V[3], c->V[8], t.chunk)

@@ -181,21 +181,32 @@ __global__ void k_k2(const c128* __restrict__ sig, const double* __restrict__ Dx
   }
 }
 
-// Compact k^2 for the stencil (OpArgs::k2im / k2re): the imaginary parts
-// of k2x, k2y, k2z, and the real parts of the i = 0, j = 0 and k = 0 slices
-// as 2-D tables rx(j,k), ry(i,k), rz(i,j).  For a sigma-hat whose imaginary
-// part is the same in every cell (model R), each edge grid's real part is
-// that value times metric weights of the two transverse axes
-// (gen_k2_{x,y,z}grid, src/aphid.f90:1306-1440), so the tables hold every
-// real part; k_k2c_check verifies this entry by entry.
-__global__ void k_k2c_extract(const c128* __restrict__ k2, double* __restrict__ im,
-                              double* __restrict__ rx, double* __restrict__ ry,
-                              double* __restrict__ rz, Dims d) {
+// Compact k^2 for the stencil (OpArgs::k2rec / k2re / k2pc): per node the
+// pairs {k2x.im, k2p.im} (xp[2q], xp[2q+1]) and {k2y.im, k2z.im}
+// (yz[2q], yz[2q+1]); the real parts of the i = 0,
+// j = 0 and k = 0 slices of the edge grids as 2-D tables rx(j,k), ry(i,k),
+// rz(i,j); and per node one byte c with k2p.re = the double whose bit
+// pattern is pbase + c (modulo 2^64).  For a sigma-hat whose imaginary part e is the same
+// in every cell (model R), each edge grid's real part is e times metric
+// weights of the two transverse axes (gen_k2_{x,y,z}grid,
+// src/aphid.f90:1306-1440), so the tables hold every real part; and k2p's
+// real part is -omega mu0 e times (sum of the eight cell weights) / (their
+// total) (gen_k2_pgrid, :1465-1531), a ratio that is 1 up to rounding, so
+// every node's value lies within a few units in the last place of every
+// other's.  k_k2c_check verifies both, entry by entry.
+__global__ void k_k2c_extract(const c128* __restrict__ k2, double* __restrict__ xp,
+                              double* __restrict__ yz, double* __restrict__ rx, double* __restrict__ ry,
+                              double* __restrict__ rz, unsigned char* __restrict__ pcode,
+                              unsigned long long pbase, Dims d) {
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < d.nn;
        q += (long long)gridDim.x * blockDim.x) {
     int i, j, k;
     d.ijk(q, i, j, k);
-    for (int c = 0; c < 3; ++c) im[c * d.nn + q] = k2[c * d.nn + q].im;
+    xp[2 * q] = k2[q].im;
+    xp[2 * q + 1] = k2[3 * d.nn + q].im;
+    yz[2 * q] = k2[d.nn + q].im;
+    yz[2 * q + 1] = k2[2 * d.nn + q].im;
+    pcode[q] = (unsigned char)((unsigned long long)__double_as_longlong(k2[3 * d.nn + q].re) - pbase);
     if (i == 0) rx[j * d.nz + k] = k2[q].re;
     if (j == 0) ry[i * d.nz + k] = k2[d.nn + q].re;
     if (k == 0) rz[i * d.ny + j] = k2[2 * d.nn + q].re;
@@ -203,10 +214,11 @@ __global__ void k_k2c_extract(const c128* __restrict__ k2, double* __restrict__ 
 }
 
 // Sets *bad = 1 if any real part of an edge grid (padding excluded) differs
-// from its table entry in any bit.
+// from its table entry in any bit, or any k2p real part's bit pattern lies
+// outside pbase + [0, 255] (modulo 2^64).
 __global__ void k_k2c_check(const c128* __restrict__ k2, const double* __restrict__ rx,
-                            const double* __restrict__ ry, const double* __restrict__ rz, Dims d,
-                            int* bad) {
+                            const double* __restrict__ ry, const double* __restrict__ rz,
+                            unsigned long long pbase, Dims d, int* bad) {
   bool diff = false;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < d.nn;
        q += (long long)gridDim.x * blockDim.x) {
@@ -220,6 +232,9 @@ __global__ void k_k2c_check(const c128* __restrict__ k2, const double* __restric
     if (k < d.nz - 1 &&
         __double_as_longlong(k2[2 * d.nn + q].re) != __double_as_longlong(rz[i * d.ny + j]))
       diff = true;
+    const unsigned long long c =
+        (unsigned long long)__double_as_longlong(k2[3 * d.nn + q].re) - pbase;
+    if (c > 255ull) diff = true;
   }
   if (diff) *bad = 1;  // every writer stores the same value
 }

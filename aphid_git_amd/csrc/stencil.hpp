@@ -43,18 +43,26 @@ struct OpArgs {
   const R* dN[3];
   const R* rdN[3];
   // Compact k^2 (complex128 x-march; nullptr: off).  When every cell's
-  // sigma-hat has the same imaginary part (model R: sigma + i omega eps0),
-  // the real parts of the edge grids depend on two coordinates only: the
-  // transverse metric weights of gen_k2_{x,y,z}grid (src/aphid.f90:1306-1440)
-  // times that one value.  The stencil then reads the imaginary parts of
-  // k2x, k2y, k2z (k2im, 3*nn values, 8 bytes each) and their real parts
-  // from the 2-D tables k2re[0](j,k) = [j*nz + k], k2re[1](i,k) = [i*nz + k],
-  // k2re[2](i,j) = [i*ny + j]; k2p stays whole.  The tables are taken from
-  // the stored grids and checked against every stored real part bit for bit
-  // before they are used (k_k2c_extract, k_k2c_check), so the stencil sees
-  // the same complex values either way.
-  const R* k2im = nullptr;
+  // sigma-hat has the same imaginary part e (model R: sigma + i omega eps0),
+  // the real parts of the edge grids depend on two coordinates only (the
+  // transverse metric weights of gen_k2_{x,y,z}grid, src/aphid.f90:1306-1440,
+  // times e), and the real part of the node grid k2p is -omega mu0 e times
+  // a weighted mean that is 1 up to rounding (gen_k2_pgrid, :1465-1531): its
+  // values lie within a few units in the last place of each other.  The
+  // stencil then reads two 16-byte pairs per node, k2rec[2q .. 2q+1] =
+  // {k2x.im, k2p.im} and, k2yz_off bytes further, the pair {k2y.im, k2z.im},
+  // and one byte k2pc[q]; k2p's real part is the double with the bit
+  // pattern k2p_base + k2pc[q], and the edge grids' real parts come from
+  // the 2-D tables k2re[0](j,k) = [j*nz + k],
+  // k2re[1](i,k) = [i*nz + k], k2re[2](i,j) = [i*ny + j].  Every real part
+  // is checked against the stored grids bit for bit before the compact form
+  // is used (k_k2c_check), so the stencil sees the same complex values
+  // either way.
+  const R* k2rec = nullptr;
+  unsigned k2yz_off = 0;
   const R* k2re[3] = {nullptr, nullptr, nullptr};
+  const unsigned char* k2pc = nullptr;
+  unsigned long long k2p_base = 0;
 };
 
 // Optional epilogue of the apply: eps = in^T out (bilinear, FP64) reduced
@@ -365,39 +373,54 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
   const int hl = (hj + 1) * LK + (hk + 1);
   const int ol = (tj + 1) * LK + (tk + 1);
 
-  // k^2 grid c (0..2) of the own column at plane i: whole, or (CK) the
-  // imaginary part and the real part from its 2-D table (0 outside the
-  // domain).  In the edge grids' padding (i = nx-1 for k2x, j = ny-1 for
-  // k2y, k = nz-1 for k2z) the table gives a nonzero real part where the
+  // Compact k^2 (CK): the pairs of node (i, column) are at byte i*plb +
+  // colb of k2rec (half = 0) and k2yz_off further (half = k2yz_off), its k2p code at
+  // byte (i*plb + colb)/16 of k2pc; the first pair {k2x.im, k2p.im} and
+  // the code are read one plane ahead (the x-queues), the second pair
+  // {k2y.im, k2z.im} at the plane itself (0 outside the domain).  Real parts: k2x from its table
+  // (one value per column, rex), k2y and k2z from theirs, k2p from its code
+  // (kpre).  In the edge grids' padding (i = nx-1 for k2x, j = ny-1 for
+  // k2y, k = nz-1 for k2z) a table gives a nonzero real part where the
   // whole grid holds 0; rows4 never reads those entries.  The ry and rz
   // tables are contiguous (k2re[2] = k2re[1] + nx*nz), so both are read
   // through one base with 32-bit byte offsets.
   constexpr unsigned RS = sizeof(R);
   const unsigned rzb = (unsigned)nx * nz * RS;
   const R rex = (CK && own) ? ldo(op.k2re[0], ((unsigned)j * nz + k) * RS) : R(0);
-  auto ldk = [&](int c, int i) -> C {
-    if constexpr (!CK) {
-      return ldc(op.k2, c, i);
-    } else {
-      if (!(own && i >= 0 && i < nx)) return C{R(0), R(0)};
-      const R im = ldo(op.k2im, (c * nnb + (unsigned)i * plb + colb) / 2u);
-      const R re = c == 0 ? rex
-                          : ldo(op.k2re[1], c == 1 ? ((unsigned)i * nz + k) * RS
-                                                   : rzb + ((unsigned)i * ny + j) * RS);
-      return C{re, im};
-    }
+  const unsigned ryo = (unsigned)min(k, nz - 1) * RS, rzo = (unsigned)min(j, ny - 1) * RS;
+  auto ldr = [&](int i, unsigned half) -> C {
+    return (own && i >= 0 && i < nx)
+               ? ldo(reinterpret_cast<const C*>(op.k2rec), half + (unsigned)i * plb + colb)
+               : C{R(0), R(0)};
   };
-  // the k^2 value a halo cell stages (grid hc of the cell at (gj, gk))
-  auto ldh = [&](unsigned g, int i) -> C {
+  // k2p's real part from its code
+  auto kpre = [&](unsigned code) -> R {
+    return __longlong_as_double((long long)(op.k2p_base + code));
+  };
+  // compact k^2, halo cell: byte offset of its real part's table entry at
+  // plane 0 (hto; hc = 3 reads the entry of k = 0, unused)
+  const unsigned hto = hc == 1 ? (unsigned)gk * RS : (hc == 2 ? rzb + (unsigned)gj * RS : 0u);
+  // k^2 grids 1 and 2 of the own column at plane i (whole k^2 only)
+  auto ldk = [&](int c, int i) -> C { return ldc(op.k2, c, i); };
+  // the k^2 value a halo cell stages (grid hc of the cell at (gj, gk));
+  // hcode: the cell's k2p code at plane i (compact k^2, hc = 3)
+  auto ldh = [&](unsigned g, int i, unsigned hcode) -> C {
     if constexpr (!CK) {
       return ldo(op.k2, hc * nnb + g);
     } else {
-      if (hc == 3) return ldo(op.k2, 3 * nnb + g);
-      const R im = ldo(op.k2im, (hc * nnb + g) / 2u);
-      const R re = ldo(op.k2re[1], hc == 1 ? ((unsigned)i * nz + gk) * RS
-                                           : rzb + ((unsigned)i * ny + gj) * RS);
-      return C{re, im};
+      // the pairs' k2p.im, k2y.im or k2z.im and the table entry (k2y, k2z),
+      // both loaded (in bounds for every hc) before one wait
+      const R im = ldo(op.k2rec, g + (hc == 3 ? 8u : (hc == 1 ? op.k2yz_off : op.k2yz_off + 8u)));
+      const R tre = ldo(op.k2re[1], (unsigned)i * (hc == 2 ? (unsigned)ny : (unsigned)nz) * RS + hto);
+      return C{hc == 3 ? kpre(hcode) : tre, im};
     }
+  };
+  // a halo cell's k2p code at plane i (compact k^2), loaded at the end of
+  // the step before: loaded next to its use, the compiler issues it only
+  // for hc = 3, after a wait for the other halo loads
+  auto ldhc = [&](int i) -> unsigned {
+    if constexpr (CK) return op.k2pc[((unsigned)min(i, nx - 1) * plb + hcolb) / CS];
+    else return 0u;
   };
 
   C pa[4], ca[4], na[4];
@@ -410,29 +433,64 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
   // the x-neighbour queue of k2x: whole values, or (CK) imaginary parts
   // only, all sharing the column's table value rex
   using KX = std::conditional_t<CK, R, C>;
-  auto ldkx = [&](int i) -> KX {
-    if constexpr (CK) return (own && i >= 0 && i < nx) ? ldo(op.k2im, (unsigned)i * (plb / 2u) + colb / 2u) : R(0);
-    else return ldk(0, i);
-  };
   auto kxv = [&](KX v) -> C {
     if constexpr (CK) return C{rex, v};
     else return v;
   };
-  KX k2x_p = ldkx(i0 - 1), k2x_c = ldkx(i0);
-  C k2p_c = ldc(K2, 3, i0);
+  // k2x and k2p of the own column at plane i (the queues' loads); with
+  // the compact k^2, k2p's real part stays a code (kc) until the value is
+  // used (kfix): a conversion next to the load would make the wave wait
+  // for every load issued before it, the plane i+1 fields included
+  auto ldxp = [&](int i, KX& kx, C& kp, unsigned& kc) {
+    if constexpr (CK) {
+      const C r = ldr(i, 0u);
+      kx = r.re;
+      kp = C{R(0), r.im};
+      // unconditional, from an in-bounds node (plane nx-1 past the last
+      // plane, column 0 for threads off the grid): a load under a branch
+      // would have its merge value formed, and waited for, at once
+      kc = op.k2pc[((unsigned)min(i, nx - 1) * plb + colb) / CS];
+    } else {
+      kx = ldc(K2, 0, i);
+      kp = ldc(K2, 3, i);
+    }
+  };
+  auto kfix = [&](C& kp, unsigned kc, int i) {
+    if constexpr (CK) kp.re = (own && i >= 0 && i < nx) ? kpre(kc) : R(0);
+  };
+  KX k2x_p, k2x_c;
+  C k2p_c;
+  unsigned kc_c = 0;
+  if constexpr (CK) k2x_p = ldr(i0 - 1, 0u).re;
+  else k2x_p = ldc(K2, 0, i0 - 1);
+  ldxp(i0, k2x_c, k2p_c, kc_c);
+  kfix(k2p_c, kc_c, i0);
+  unsigned hc_c = ldhc(i0);
 
   double eacc_re = 0.0, eacc_im = 0.0;
 
   for (int i = i0; i < i1; ++i) {
     // stage plane i: the loads this step needs (own k2y, k2z and the
     // halo) go out first, since waits on loads complete in issue order
-    sm[4][ol] = ldk(1, i);
-    sm[5][ol] = ldk(2, i);
+    if constexpr (CK) {
+      // the pair and both table entries are loaded unconditionally, from
+      // in-bounds addresses (column 0 and clamped table indices off the
+      // grid), and masked afterwards, so that all three go out before one
+      // wait
+      const C r = ldo(reinterpret_cast<const C*>(op.k2rec), op.k2yz_off + (unsigned)i * plb + colb);
+      const R ty = ldo(op.k2re[1], (unsigned)i * nz * RS + ryo);
+      const R tz = ldo(op.k2re[1], rzb + (unsigned)i * ny * RS + rzo);
+      sm[4][ol] = own ? C{ty, r.re} : C{R(0), R(0)};
+      sm[5][ol] = own ? C{tz, r.im} : C{R(0), R(0)};
+    } else {
+      sm[4][ol] = ldk(1, i);
+      sm[5][ol] = ldk(2, i);
+    }
     if (hv) {
       const unsigned g = (unsigned)i * plb + hcolb;
 #pragma unroll
       for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : C{R(0), R(0)};
-      sm[3 + hc][hl] = hin ? ldh(g, i) : C{R(0), R(0)};
+      sm[3 + hc][hl] = hin ? ldh(g, i, hc_c) : C{R(0), R(0)};
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
@@ -440,9 +498,12 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
     // plane i+1 of the own column (the x-neighbours of the next step)
 #pragma unroll
     for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
-    const KX k2x_n = ldkx(i + 1);
-    const C k2p_n = ldc(K2, 3, i + 1);
+    KX k2x_n;
+    C k2p_n;
+    unsigned kc_n = 0;
+    ldxp(i + 1, k2x_n, k2p_n, kc_n);
     __syncthreads();
+    kfix(k2p_n, kc_n, i + 1);
 
     if (own) {
       const unsigned ob = (unsigned)i * plb + colb;
@@ -477,6 +538,7 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
     k2x_p = k2x_c;
     k2x_c = k2x_n;
     k2p_c = k2p_n;
+    hc_c = ldhc(i + 1);
   }
 
   if constexpr (!SEG) {
@@ -555,39 +617,55 @@ __global__ __launch_bounds__(TK* TJ) void k_apply_traffic(OpArgs<R> op,
     return (own && i >= 0 && i < nx) ? ldo(base, c * nnb + (unsigned)i * plb + colb)
                                      : C{R(0), R(0)};
   };
-  // k^2 grids 0..2 as k_apply<..., CK> reads them (imaginary part + table)
-  auto ldk = [&](int c, int i) -> C {
+  // the k^2 loads of k_apply<..., CK>: whole grids, or the compact
+  // record halves, k2p codes and tables
+  auto ldr = [&](int i, unsigned half) -> C {
+    return (own && i >= 0 && i < nx)
+               ? ldo(reinterpret_cast<const C*>(op.k2rec), half + (unsigned)i * plb + colb)
+               : C{R(0), R(0)};
+  };
+  // {k2x, k2p} of plane i
+  auto ldxp = [&](int i) -> C {
     if constexpr (!CK) {
-      return ldc(K2, c, i);
+      return ldc(K2, 0, i) + ldc(K2, 3, i);
     } else {
-      if (!(own && i >= 0 && i < nx)) return C{R(0), R(0)};
-      const R im = ldo(op.k2im, (c * nnb + (unsigned)i * plb + colb) / 2u);
-      const R re = c == 0 ? op.k2re[0][j * nz + k]
-                          : (c == 1 ? op.k2re[1][i * nz + k] : op.k2re[2][i * ny + j]);
-      return C{re, im};
+      const C r = ldr(i, 0u);
+      if (!(own && i >= 0 && i < nx)) return r;
+      return C{r.re + R(op.k2pc[((unsigned)i * plb + colb) / CS]), r.im};
+    }
+  };
+  // {k2y, k2z} of plane i
+  auto ldyz = [&](int i) -> C {
+    if constexpr (!CK) {
+      return ldc(K2, 1, i) + ldc(K2, 2, i);
+    } else {
+      const C r = ldr(i, op.k2yz_off);
+      if (!own) return r;
+      return C{r.re + op.k2re[1][i * nz + k], r.im + op.k2re[2][i * ny + j]};
     }
   };
   auto ldh = [&](unsigned g, int i) -> C {
     if constexpr (!CK) {
       return ldo(K2, hc * nnb + g);
     } else {
-      if (hc == 3) return ldo(K2, 3 * nnb + g);
-      const R im = ldo(op.k2im, (hc * nnb + g) / 2u);
+      if (hc == 3) return C{ldo(op.k2rec, g + 8u), R(op.k2pc[g / CS])};
+      const R im = ldo(op.k2rec, g + (hc == 1 ? op.k2yz_off : op.k2yz_off + 8u));
       const R re = hc == 1 ? op.k2re[1][i * nz + gk] : op.k2re[2][i * ny + gj];
       return C{re, im};
     }
   };
   hold[threadIdx.x] = C{R(0), R(0)};
-  C acc = ldk(0, i0 - 1);
+  C acc = ldxp(i0 - 1);
+  if constexpr (CK) acc.re += own ? op.k2re[0][j * nz + k] : R(0);
   C ca[4], na[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     acc += ldc(in, c, i0 - 1);
     ca[c] = ldc(in, c, i0);
   }
-  C kx = ldk(0, i0), kp = ldc(K2, 3, i0);
+  C kx = ldxp(i0), kp = {R(0), R(0)};
   for (int i = i0; i < i1; ++i) {
-    const C ky = ldk(1, i), kz = ldk(2, i);
+    const C ky = ldyz(i), kz = {R(0), R(0)};
     C h = {R(0), R(0)};
     if (hin) {
       const unsigned g = (unsigned)i * plb + hcolb;
@@ -597,7 +675,7 @@ __global__ __launch_bounds__(TK* TJ) void k_apply_traffic(OpArgs<R> op,
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) na[c] = ldc(in, c, i + 1);
-    const C kxn = ldk(0, i + 1), kpn = ldc(K2, 3, i + 1);
+    const C kxn = ldxp(i + 1), kpn = {R(0), R(0)};
     if (own) {
       const unsigned ob = (unsigned)i * plb + colb;
       const C s = acc + h + kx + kp + ky + kz;

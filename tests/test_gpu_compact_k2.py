@@ -1,6 +1,7 @@
 """The stencil's compact k^2 (aphid_set_compact_k2): for models whose
 sigma-hat has one imaginary part in every cell (model R), the real parts of
-k2x, k2y and k2z come from 2-D tables and only the imaginary parts are read
+k2x, k2y and k2z come from 2-D tables, the real part of k2p from a one-byte
+offset of its bit pattern, and a 32-byte record of the four imaginary parts
 per node.  The operator's values must not change: the compact and the whole
 k^2 give the same applies and the same solves, bit for bit, and the library
 keeps the whole grids wherever the tables would not reproduce them."""
@@ -93,7 +94,9 @@ def test_compact_k2_only_where_the_tables_hold(gpu, oracle, model, active):
 
 def test_compact_k2_from_host_k2(gpu, oracle):
     """k^2 given through aphid_set_k2: the oracle's model-R grids keep the
-    structure (compact), the same grids with one real part changed do not."""
+    structure (compact), the same grids with one edge real part changed do
+    not, nor do grids whose k2p real parts are spread wider than 256 units
+    in the last place."""
     case = meshgen.random_block_case((14, 12, 10), seed=9, freq=7.0)
     gpu.set_mesh(case.xn, case.yn, case.zn)
     S = oracle.System(case)
@@ -107,3 +110,26 @@ def test_compact_k2_from_host_k2(gpu, oracle):
     assert not gpu.set_compact_k2(True)
     b = flat4(gpu.apply(u))
     assert relerr(b, a) > 0  # the changed entry is used
+    k2[1][3, 4, 5] = S.k2[1][3, 4, 5]
+    k2[3][2, 2, 2] *= 1.0 + 1e-12
+    gpu.set_k2(k2)
+    assert not gpu.set_compact_k2(True)
+    c = flat4(gpu.apply(u))
+    assert relerr(c, a) > 0
+
+
+def test_compact_k2_follows_the_frequency(gpu):
+    """k2p's real part depends on omega: a context rebuilt at another
+    frequency must use the new value (the applies stay bitwise those of the
+    whole k^2)."""
+    for freq in (3.0, 250.0, 3.0e4):
+        case = meshgen.random_block_case((12, 10, 9), seed=4, freq=freq)
+        gpu.set_mesh(case.xn, case.yn, case.zn)
+        gpu.set_cells(case.model, case.omega, case.cells)
+        assert gpu.set_compact_k2(True), freq
+        u = _rand4(case.shape, 6)
+        a = flat4(gpu.apply(u))
+        assert not gpu.set_compact_k2(False)
+        b = flat4(gpu.apply(u))
+        gpu.set_compact_k2(True)
+        assert _bitwise(a, b), freq
