@@ -363,9 +363,10 @@ class Solver:
         _check(lib().aphid_set_reproducible(self._ctx, int(bool(on))), "aphid_set_reproducible")
 
     def set_compact_k2(self, on: bool = True) -> bool:
-        """Compact complex128 k^2 in the stencil (imaginary parts plus 2-D
-        tables of the real parts, used only where exact); returns whether it
-        is in use."""
+        """Compact complex128 k^2 in the stencil (imaginary parts as two
+        pair arrays, k2p's real part as a one-byte offset of its bit pattern,
+        2-D tables of the edge grids' real parts; used only where exact);
+        returns whether it is in use."""
         a = C.c_int()
         _check(lib().aphid_set_compact_k2(self._ctx, int(bool(on)), C.byref(a)),
                "aphid_set_compact_k2")

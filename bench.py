@@ -608,16 +608,20 @@ def main():
                              "frac_of_peak": B_mv / t_traffic / 1e9 / HBM_PEAK_GBS,
                              "plain_apply_over_ceiling": t_traffic / t_apply,
                              "definition": "aphid_time_traffic: the tuned march tile's loads "
-                                           "(own columns of 4 fields and 4 k^2 grids, halo "
-                                           "cells) and 4 stores per node, no LDS, barriers "
-                                           "or arithmetic, same occupancy"},
+                                           "(own columns of the 4 fields and of k^2 as the "
+                                           "stencil reads it, halo cells) and 4 stores per "
+                                           "node, no LDS, barriers or arithmetic, same "
+                                           "occupancy"},
                          "tile": {"tk": tile[0], "tj": tile[1], "chunk": tile[2]},
                          "k2_compact": k2_compact,
                          "k2_compact_note": "with the compact k^2 (model R) the stencil reads "
-                                            "the edge grids k2x, k2y, k2z as 8-byte imaginary "
-                                            "parts plus 2-D tables of their real parts (the "
-                                            "same values, checked bit for bit), so it moves "
-                                            "2.625*M*16 of the 3*M*16 algorithmic bytes; "
+                                            "33 bytes of k^2 per node instead of 64: the "
+                                            "imaginary parts of k2x, k2p, k2y, k2z as two "
+                                            "16-byte pairs, a one-byte offset of k2p's real "
+                                            "part from a common bit pattern, and the edge "
+                                            "grids' real parts from 2-D tables (the same "
+                                            "values, checked bit for bit), so it moves about "
+                                            "2.52*M*16 of the 3*M*16 algorithmic bytes; "
                                             "achieved and frac count the algorithmic bytes, "
                                             "traffic is what the counters measured"},
             "roofline_dominant_kernel": {"kernel": dom, **kernels[dom], "peak": HBM_PEAK_GBS},

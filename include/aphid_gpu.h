@@ -214,10 +214,13 @@ int aphid_set_reproducible(aphid_ctx *ctx, int on);
  * src/aphid.f90:803-1282, that leaves every value it computes unchanged).
  * When every cell's sigma-hat has the same imaginary part (model R:
  * sigma + i omega eps0, src/aphid.f90:379-449), the real parts of the edge
- * grids k2x, k2y, k2z depend on two coordinates only; the stencil then
- * reads their imaginary parts (8 bytes instead of 16) and takes the real
- * parts from 2-D tables.  The library checks every real part against the
- * tables, bit for bit, each time k^2 is set, and uses the compact form only
+ * grids k2x, k2y, k2z depend on two coordinates only, and the real parts of
+ * the node grid k2p lie within a few units in the last place of each
+ * other; the stencil then reads 33 bytes of k^2 per node instead of 64:
+ * the four imaginary parts as two 16-byte pairs, k2p's real part as a
+ * one-byte offset of its bit pattern from a common base, and the edge
+ * grids' real parts from 2-D tables.  The library checks every real part
+ * against this form, bit for bit, each time k^2 is set, and uses it only
  * if all agree (so models C, D and I, or k^2 given by aphid_set_k2 without
  * that structure, keep the whole grids).  on = 0 switches it off (the
  * stencil then reads the whole grids; the results are the same bits).
