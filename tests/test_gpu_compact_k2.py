@@ -42,6 +42,10 @@ def test_compact_k2_applies_are_bitwise_the_whole_k2_applies_on_config2(gpu):
     print(f"[compact] config2 {case.shape}", flush=True)
     gpu.setup_case(case)
     assert gpu.set_compact_k2(True), "model R must take the compact k^2"
+    kp = np.ascontiguousarray(np.asarray(gpu.get_k2()[3]).real).view(np.int64)
+    spread = int(kp.max() - kp.min())
+    print(f"[compact] config2 k2p real parts span {spread} units in the last place", flush=True)
+    assert spread <= 255
     u = _rand4(case.shape, 3)
     tiles = [None, (32, 8, 12), (16, 16, 8)]
     for t in tiles:

@@ -257,3 +257,33 @@ def test_unet_reference_gradient_matches_finite_differences():
             assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(g[q])) + 1e-9, (q, fd, g[q])
     finally:
         unet_ref.rb, unet_ref.gb = saved
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+@pytest.mark.parametrize("freq", [1.0, 250.0, 3.0e4])
+def test_model_r_k2_structure_behind_the_compact_form(oracle, freq):
+    """What the stencil's compact k^2 relies on, on the oracle's model-R
+    grids (gen_k2_*grid, src/aphid.f90:1306-1531): the real part of each
+    edge grid is the same bit pattern along that grid's own axis (it
+    depends on the two transverse coordinates only), and the real parts of
+    k2p lie within a few units in the last place of each other (-omega mu0
+    times a weighted mean of one value).  Per-cell imaginary parts (model C)
+    break the first."""
+    # random spacings on every axis (tiny_case), so that the weights differ
+    case = meshgen.tiny_case(13, 11, 9, seed=3, model="R", freq=freq, stretch=2.0)
+    kx, ky, kz, kp = (np.asarray(a) for a in oracle.System(case).k2)
+    for a, axis in ((kx, 0), (ky, 1), (kz, 2)):
+        re = _bits(a.real)
+        first = np.take(re, [0], axis=axis)
+        assert np.array_equal(re, np.broadcast_to(first, re.shape)), axis
+    pre = _bits(kp.real)
+    spread = int(pre.max() - pre.min())
+    print(f"[k2 structure] {freq} Hz: k2p real parts span {spread} units in the last place")
+    assert np.all(kp.real < 0) and spread <= 16
+    c = meshgen.tiny_case(9, 8, 7, seed=11, model="C", freq=freq)
+    cx = np.asarray(oracle.System(c).k2[0])
+    re = _bits(cx.real)
+    assert not np.array_equal(re, np.broadcast_to(re[:1], re.shape))
