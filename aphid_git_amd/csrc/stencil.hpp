@@ -398,10 +398,10 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
     return __longlong_as_double((long long)(op.k2p_base + code));
   };
   // compact k^2, halo cell: byte offset of its real part's table entry at
-  // plane 0 (hto; hc = 3 reads the entry of k = 0, unused)
-  const unsigned hto = hc == 1 ? (unsigned)gk * RS : (hc == 2 ? rzb + (unsigned)gj * RS : 0u);
-  // k^2 grids 1 and 2 of the own column at plane i (whole k^2 only)
-  auto ldk = [&](int c, int i) -> C { return ldc(op.k2, c, i); };
+  // plane 0 (hto; hc = 3 reads the entry of k = 0, unused; a cell off the
+  // grid reads a clamped, in-bounds entry)
+  const unsigned hto = hc == 1 ? (unsigned)min(max(gk, 0), nz - 1) * RS
+                               : (hc == 2 ? rzb + (unsigned)min(max(gj, 0), ny - 1) * RS : 0u);
   // the k^2 value a halo cell stages (grid hc of the cell at (gj, gk));
   // hcode: the cell's k2p code at plane i (compact k^2, hc = 3)
   auto ldh = [&](unsigned g, int i, unsigned hcode) -> C {
@@ -483,14 +483,21 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
       sm[4][ol] = own ? C{ty, r.re} : C{R(0), R(0)};
       sm[5][ol] = own ? C{tz, r.im} : C{R(0), R(0)};
     } else {
-      sm[4][ol] = ldk(1, i);
-      sm[5][ol] = ldk(2, i);
+      sm[4][ol] = ldc(op.k2, 1, i);
+      sm[5][ol] = ldc(op.k2, 2, i);
     }
     if (hv) {
+      // the halo cell's five loads go out together, unconditionally (a cell
+      // off the grid reads node (i, 0, 0), hcolb = 0) and are masked
+      // afterwards: a load under its own branch gets its own wait
       const unsigned g = (unsigned)i * plb + hcolb;
+      C hf[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? ldo(in, c * nnb + g) : C{R(0), R(0)};
-      sm[3 + hc][hl] = hin ? ldh(g, i, hc_c) : C{R(0), R(0)};
+      for (int c = 0; c < 4; ++c) hf[c] = ldo(in, c * nnb + g);
+      const C hk2 = ldh(g, i, hc_c);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm[c][hl] = hin ? hf[c] : C{R(0), R(0)};
+      sm[3 + hc][hl] = hin ? hk2 : C{R(0), R(0)};
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm[c][ol] = ca[c];
