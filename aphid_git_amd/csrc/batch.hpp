@@ -313,24 +313,37 @@ __global__ __launch_bounds__(NT) void kb_apply(BatchDev B, const OpArgs<R>* __re
     C k2x_p = ldc(K2, 0, i0 - 1), k2x_c = ldc(K2, 0, i0), k2p_c = ldc(K2, 3, i0);
     for (int i = i0; i < i1; ++i) {
       // stage plane i: own values, then the window's halo (nz nodes below
-      // the chunk: fields, k2y, k2z; nz above: fields, k2p)
-      sm[4 * ws + w] = ldc(K2, 1, i);
-      sm[5 * ws + w] = ldc(K2, 2, i);
+      // the chunk: fields, k2y, k2z; nz above: fields, k2p).  The loads go
+      // out unconditionally, from in-bounds addresses (node (i, 0, 0) off
+      // the system's plane), and are masked afterwards: a load under its
+      // own branch gets its own wait before the next is issued
+      // (own k2y and k2z are stored after the halo, so that their loads and
+      // the first halo cell's share one wait)
+      const unsigned go = (unsigned)i * plb + colb;
+      const C ky = ldg(K2, nnb + go), kz = ldg(K2, 2 * nnb + go);
       for (int h = threadIdx.x; h < 2 * nz; h += NT) {
         const bool lo = h < nz;
         const int hq = lo ? q0 - nz + h : q0 + NT + (h - nz);
         const int hw = lo ? h : NT + h;
         const bool hin = hq >= 0 && hq < pl;
         const unsigned g = (unsigned)i * plb + (hin ? (unsigned)hq * CS : 0u);
+        C hf[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sm[c * ws + hw] = hin ? ldg(sin, c * nnb + g) : C{R(0), R(0)};
+        for (int c = 0; c < 4; ++c) hf[c] = ldg(sin, c * nnb + g);
+        // below: k2y and k2z; above: k2p (twice, the second unused)
+        const C ha = ldg(K2, (lo ? nnb : 3 * nnb) + g);
+        const C hb = ldg(K2, (lo ? 2 * nnb : 3 * nnb) + g);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sm[c * ws + hw] = hin ? hf[c] : C{R(0), R(0)};
         if (lo) {
-          sm[4 * ws + hw] = hin ? ldg(K2, nnb + g) : C{R(0), R(0)};
-          sm[5 * ws + hw] = hin ? ldg(K2, 2 * nnb + g) : C{R(0), R(0)};
+          sm[4 * ws + hw] = hin ? ha : C{R(0), R(0)};
+          sm[5 * ws + hw] = hin ? hb : C{R(0), R(0)};
         } else {
-          sm[6 * ws + hw] = hin ? ldg(K2, 3 * nnb + g) : C{R(0), R(0)};
+          sm[6 * ws + hw] = hin ? ha : C{R(0), R(0)};
         }
       }
+      sm[4 * ws + w] = own ? ky : C{R(0), R(0)};
+      sm[5 * ws + w] = own ? kz : C{R(0), R(0)};
 #pragma unroll
       for (int c = 0; c < 4; ++c) sm[c * ws + w] = ca[c];
       sm[6 * ws + w] = k2p_c;
