@@ -373,17 +373,18 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
   const int hl = (hj + 1) * LK + (hk + 1);
   const int ol = (tj + 1) * LK + (tk + 1);
 
-  // Compact k^2 (CK): the pairs of node (i, column) are at byte i*plb +
-  // colb of k2rec (half = 0) and k2yz_off further (half = k2yz_off), its k2p code at
-  // byte (i*plb + colb)/16 of k2pc; the first pair {k2x.im, k2p.im} and
-  // the code are read one plane ahead (the x-queues), the second pair
-  // {k2y.im, k2z.im} at the plane itself (0 outside the domain).  Real parts: k2x from its table
-  // (one value per column, rex), k2y and k2z from theirs, k2p from its code
-  // (kpre).  In the edge grids' padding (i = nx-1 for k2x, j = ny-1 for
-  // k2y, k = nz-1 for k2z) a table gives a nonzero real part where the
-  // whole grid holds 0; rows4 never reads those entries.  The ry and rz
-  // tables are contiguous (k2re[2] = k2re[1] + nx*nz), so both are read
-  // through one base with 32-bit byte offsets.
+  // Compact k^2 (CK): the pairs of node (i, column) are at byte
+  // i*plb + colb of k2rec (half = 0) and k2yz_off bytes further
+  // (half = k2yz_off), its k2p code at byte (i*plb + colb)/16 of k2pc.  The
+  // first pair {k2x.im, k2p.im} and the code are read one plane ahead (the
+  // x-queues), the second pair {k2y.im, k2z.im} at the plane itself (0
+  // outside the domain).  Real parts: k2x from its table (one value per
+  // column, rex), k2y and k2z from theirs, k2p from its code (kpre).  In
+  // the edge grids' padding (i = nx-1 for k2x, j = ny-1 for k2y, k = nz-1
+  // for k2z) a table gives a nonzero real part where the whole grid holds
+  // 0; rows4 never reads those entries.  The ry and rz tables are
+  // contiguous (k2re[2] = k2re[1] + nx*nz), so both are read through one
+  // base with 32-bit byte offsets.
   constexpr unsigned RS = sizeof(R);
   const unsigned rzb = (unsigned)nx * nz * RS;
   const R rex = (CK && own) ? ldo(op.k2re[0], ((unsigned)j * nz + k) * RS) : R(0);
@@ -408,8 +409,8 @@ __global__ __launch_bounds__(TK* TJ, CK ? 3 : 1) void k_apply(OpArgs<R> op, cons
     if constexpr (!CK) {
       return ldo(op.k2, hc * nnb + g);
     } else {
-      // the pairs' k2p.im, k2y.im or k2z.im and the table entry (k2y, k2z),
-      // both loaded (in bounds for every hc) before one wait
+      // the pairs' k2p.im, k2y.im or k2z.im and the table entry (k2y,
+      // k2z), both loaded unconditionally (in bounds for every hc)
       const R im = ldo(op.k2rec, g + (hc == 3 ? 8u : (hc == 1 ? op.k2yz_off : op.k2yz_off + 8u)));
       const R tre = ldo(op.k2re[1], (unsigned)i * (hc == 2 ? (unsigned)ny : (unsigned)nz) * RS + hto);
       return C{hc == 3 ? kpre(hcode) : tre, im};
