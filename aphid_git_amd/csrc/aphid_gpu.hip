@@ -3138,6 +3138,24 @@ int batch_solve_wave(aphid_batch* B, const std::vector<int>& in_wave, double tol
 // least squares on the residual; x0 = 0 if neither fit is usable.  Runs on
 // q's operator context, which borrows q's slices of the flat vectors (x, b
 // and the spare Krylov vectors, which the solve rewrites before reading).
+// wave of every system of a sweep schedule: 0 without sources, else one
+// more than its latest source; false if the sources form a cycle
+bool sweep_levels(const aphid_batch* B, std::vector<int>& level) {
+  const int ns = (int)B->sys.size();
+  level.assign(ns, 0);
+  for (int pass = 0; pass <= ns; ++pass) {
+    bool changed = false;
+    for (int q = 0; q < ns; ++q)
+      for (int f : B->sys[q].from)
+        if (level[q] < level[f] + 1) {
+          level[q] = level[f] + 1;
+          changed = true;
+        }
+    if (!changed) return true;
+  }
+  return false;
+}
+
 int batch_fit_guess(aphid_batch* B, int q) {
   auto& sy = B->sys[q];
   aphid_ctx* c = B->ops[sy.op];
@@ -3259,29 +3277,28 @@ int aphid_batch_solve(aphid_batch* B, double tol, int max_it, unsigned flags,
   std::vector<QmrState> last(ns);
   int rc = 0;
   if (!(flags & APHID_F_MIXED)) {
-    // one wave, or a sweep: first every system without a sweep guess, then
-    // the others from initial guesses fitted to the first wave's solutions
-    std::vector<int> w0(ns), w1(ns);
-    bool sweep = false;
-    for (int q = 0; q < ns; ++q) {
-      const bool second = !B->sys[q].from.empty();
-      w0[q] = !second;
-      w1[q] = second;
-      sweep |= second;
-      if (!second) B->sys[q].r0 = use_x0 ? NAN : 1.0;
-    }
-    rc = batch_solve_wave(B, w0, tol, max_it, use_x0, last, iters);
-    if (rc) return rc;
-    if (sweep) {
-      CK(hipStreamSynchronize(B->s));
-      for (int q = 0; q < ns; ++q)
-        if (w1[q]) {
-          rc = batch_fit_guess(B, q);
-          if (rc) return rc;
-        }
-      for (int q = 0; q < ns; ++q)
-        if (w1[q]) CK(hipStreamSynchronize(B->ops[B->sys[q].op]->s));
-      rc = batch_solve_wave(B, w1, tol, max_it, true, last, iters);
+    // one wave, or a sweep: first every system without a sweep guess,
+    // then wave by wave the others, each from an initial guess fitted to
+    // the solutions of its sources, which earlier waves solved
+    std::vector<int> level;
+    if (!sweep_levels(B, level)) return fail(APHID_E_STATE, "sweep guesses form a cycle");
+    const int top = ns ? *std::max_element(level.begin(), level.end()) : 0;
+    for (int q = 0; q < ns; ++q)
+      if (level[q] == 0) B->sys[q].r0 = use_x0 ? NAN : 1.0;
+    for (int lev = 0; lev <= top; ++lev) {
+      std::vector<int> w(ns);
+      for (int q = 0; q < ns; ++q) w[q] = level[q] == lev;
+      if (lev > 0) {
+        CK(hipStreamSynchronize(B->s));
+        for (int q = 0; q < ns; ++q)
+          if (w[q]) {
+            rc = batch_fit_guess(B, q);
+            if (rc) return rc;
+          }
+        for (int q = 0; q < ns; ++q)
+          if (w[q]) CK(hipStreamSynchronize(B->ops[B->sys[q].op]->s));
+      }
+      rc = batch_solve_wave(B, w, tol, max_it, lev > 0 || use_x0, last, iters);
       if (rc) return rc;
     }
   } else {
@@ -3380,13 +3397,16 @@ int aphid_batch_set_sweep_guess(aphid_batch* B, int sys, int nfrom, const int* f
     const aphid_ctx* cf = B->ops[B->sys[f].op];
     if (cf->nx != cs->nx || cf->ny != cs->ny || cf->nz != cs->nz)
       return fail(APHID_E_ARG, "sweep guesses come from systems on the same mesh");
-    if (!B->sys[f].from.empty())
-      return fail(APHID_E_ARG, "a source system must be in the first wave");
   }
-  for (int q = 0; q < ns && nfrom > 0; ++q)
-    for (int f : B->sys[q].from)
-      if (f == sys) return fail(APHID_E_ARG, "system is a source of another system's guess");
+  // a source may itself be fitted from earlier waves, but no system may
+  // (through its sources' sources) depend on itself
+  std::vector<int> saved = B->sys[sys].from;
   B->sys[sys].from.assign(from, from + nfrom);
+  std::vector<int> level;
+  if (!sweep_levels(B, level)) {
+    B->sys[sys].from = saved;
+    return fail(APHID_E_ARG, "sweep guesses would form a cycle");
+  }
   return 0;
 }
 

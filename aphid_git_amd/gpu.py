@@ -470,8 +470,9 @@ class BatchSolver:
         _check(lib().aphid_batch_set_guess(self._b, sys, from_sys), "aphid_batch_set_guess")
 
     def set_sweep_guess(self, sys: int, from_sys: Sequence[int]):
-        """Second-wave system: its initial guess is fitted (Galerkin) to
-        the solutions of from_sys (first-wave systems on the same mesh)."""
+        """Fitted system: its initial guess is fitted (Galerkin) to the
+        solutions of from_sys (systems on the same mesh, solved in earlier
+        waves; they may be fitted systems themselves, without a cycle)."""
         arr = np.ascontiguousarray(list(from_sys), np.int32)
         _check(lib().aphid_batch_set_sweep_guess(self._b, sys, len(arr),
                                                  _p(arr) if len(arr) else None),

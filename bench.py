@@ -196,30 +196,59 @@ def bench_batch():
     B.close()
     conv_b = sum(1 for r in reps if r["status"] == 0 and r["true_rel_res"] <= TOL)
     # the combined path: batched, complex128 (mixed precision loses on this
-    # family), and a two-wave sweep -- even frequencies from x0 = 0, then
-    # each odd frequency from its neighbours' solutions (same dipole), with
-    # Galerkin-fitted amplitudes on its own operator
-    B = gpu.BatchSolver(0)
-    ops = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells) for f in freqs]
-    for o in ops:
-        for q in range(3):
-            B.add_system(o, bs[q])
-    for k in range(1, len(freqs), 2):
-        for q in range(3):
-            nb = [k - 1] + ([k + 1] if k + 1 < len(freqs) else [])
-            B.set_sweep_guess(3 * k + q, [3 * kk + q for kk in nb])
-    B.solve(TOL, 20000)  # warm
-    t0 = time.perf_counter()
-    reps_sw, _ = B.solve(TOL, 20000)
-    t_sw = time.perf_counter() - t0
-    diff = 0.0
-    for q in range(len(reps_sw)):
-        a = np.concatenate([v.ravel() for v in B.solution(q)])
-        w = np.concatenate([v.ravel() for v in x_one[q]])
-        diff = max(diff, float(np.linalg.norm(a - w) / np.linalg.norm(w)))
-    B.close()
-    conv_sw = sum(1 for r in reps_sw if r["status"] == 0 and r["true_rel_res"] <= TOL)
-    odd = [3 * k + q for k in range(1, len(freqs), 2) for q in range(3)]
+    # family), and a frequency sweep of L levels: every 2^(L-1)-th frequency
+    # from x0 = 0, then each other frequency k from the solutions at k - d
+    # and k + d (same dipole), d the lowest set bit of k, with
+    # Galerkin-fitted amplitudes on its own operator.  L = 2: even
+    # frequencies from zero, odd ones from their neighbours.  The fastest
+    # schedule is the combined figure
+    def sweep(levels):
+        B = gpu.BatchSolver(0)
+        ops = [B.add_operator(base.xn, base.yn, base.zn, "R", 2 * np.pi * f, base.cells)
+               for f in freqs]
+        for o in ops:
+            for q in range(3):
+                B.add_system(o, bs[q])
+        fitted = {}
+        for k in range(len(freqs)):
+            if k % (1 << (levels - 1)) == 0:
+                continue
+            d = k & -k
+            for q in range(3):
+                nb = [k - d] + ([k + d] if k + d < len(freqs) else [])
+                B.set_sweep_guess(3 * k + q, [3 * kk + q for kk in nb])
+                fitted[3 * k + q] = d
+        B.solve(TOL, 20000)  # warm
+        t0 = time.perf_counter()
+        rs, _ = B.solve(TOL, 20000)
+        t = time.perf_counter() - t0
+        diff = 0.0
+        for q in range(len(rs)):
+            a = np.concatenate([v.ravel() for v in B.solution(q)])
+            w = np.concatenate([v.ravel() for v in x_one[q]])
+            diff = max(diff, float(np.linalg.norm(a - w) / np.linalg.norm(w)))
+        B.close()
+        conv = sum(1 for r in rs if r["status"] == 0 and r["true_rel_res"] <= TOL)
+        odd = [q for q, d in fitted.items() if d == 1]
+        row = {"converged": conv, "seconds": t, "solves_per_s": conv / t,
+               "speedup_vs_sequential_c128": (conv / t) / (conv_seq / t_seq),
+               "speedup_vs_batched_one_wave": (conv / t) / (conv_b / t_b),
+               "iters_mean": float(np.mean([r["iters"] for r in rs])),
+               "iters_total": int(sum(r["iters"] for r in rs)),
+               "iters_fitted_from_neighbours_mean": float(np.mean([rs[q]["iters"] for q in odd])),
+               "iters_same_systems_from_zero_mean": float(np.mean([reps[q]["iters"] for q in odd])),
+               "r0_fitted_from_neighbours_median": float(np.median([rs[q]["r0_rel"] for q in odd])),
+               "max_field_rel_diff_vs_one_wave": diff}
+        row["levels"] = levels
+        row["by_distance"] = {
+            str(d): {"systems": len(qs), "iters_mean": float(np.mean([rs[q]["iters"] for q in qs])),
+                     "r0_median": float(np.median([rs[q]["r0_rel"] for q in qs]))}
+            for d in sorted(set(fitted.values()))
+            for qs in [[q for q, dd in fitted.items() if dd == d]]}
+        return row
+
+    sws = {L: sweep(L) for L in (2, 3, 4)}
+    best = min(sws.values(), key=lambda r: r["seconds"])
     return {"workload": "33x33x33-node model family (config 1), 64 frequencies 1e-2..1e5 Hz x "
                         "3 unit dipoles (x, y, z) at seeded surface nodes = 192 systems, rel-res 1e-8",
             "systems": len(reps), "converged_batched": conv_b, "converged_sequential": conv_seq,
@@ -230,16 +259,12 @@ def bench_batch():
             "iters_max": int(max(r["iters"] for r in reps)),
             "iters_mean_sequential": float(np.mean(its_seq)),
             "combined": {
-                "what": "batched x complex128 x two-wave frequency sweep (odd frequencies start "
-                        "from a Galerkin fit of their two neighbours' solutions)",
-                "converged": conv_sw, "seconds": t_sw, "solves_per_s": conv_sw / t_sw,
-                "speedup_vs_sequential_c128": (conv_sw / t_sw) / (conv_seq / t_seq),
-                "speedup_vs_batched_one_wave": (conv_sw / t_sw) / (conv_b / t_b),
-                "iters_mean": float(np.mean([r["iters"] for r in reps_sw])),
-                "iters_second_wave_mean": float(np.mean([reps_sw[q]["iters"] for q in odd])),
-                "iters_second_wave_mean_from_zero": float(np.mean([reps[q]["iters"] for q in odd])),
-                "r0_second_wave_median": float(np.median([reps_sw[q]["r0_rel"] for q in odd])),
-                "max_field_rel_diff_vs_one_wave": diff}}
+                "what": "batched x complex128 x frequency sweep of %d levels (every %d-th "
+                        "frequency from x0 = 0, each other one from a Galerkin fit of the "
+                        "solutions d frequencies below and above it, d the lowest set bit of "
+                        "its index)" % (best["levels"], 1 << (best["levels"] - 1)),
+                **best},
+            "sweeps_by_levels": {str(L): r for L, r in sws.items()}}
 
 
 def bench_batch3b(max_it: int = 10000):

@@ -304,15 +304,18 @@ int aphid_batch_get_solution(aphid_batch *batch, int sys, double *x_x,
 int aphid_batch_set_guess(aphid_batch *batch, int sys, int from_sys);
 /* Sweep schedule (no counterpart in the reference, whose farm solves every
  * system from x0 = 0, src/aphid_parallel.f90:201 -> src/aphid_qmr.f90:
- * 230-233): mark system sys as second-wave.  aphid_batch_solve then solves
- * every unmarked system first, fits sys's initial guess to the solutions of
- * the nfrom (<= APHID_SWEEP_MAX_FROM) systems `from` -- same mesh, e.g. the
- * neighbouring frequencies with the same source -- by the Galerkin
- * condition x_a^T (b - K x0) = 0 on sys's operator (least squares on the
- * residual if that is worse than x0 = 0, and x0 = 0 if both are), and
- * solves the marked systems from those guesses (an x0 given for them is
- * replaced).  Source systems must be unmarked.  nfrom = 0 unmarks.  A
- * mixed-precision batch solve (APHID_F_MIXED) ignores the marks. */
+ * 230-233): mark system sys as fitted from the nfrom
+ * (<= APHID_SWEEP_MAX_FROM) systems `from` -- same mesh, e.g. the
+ * neighbouring frequencies with the same source.  aphid_batch_solve solves
+ * in waves: first every unmarked system, then each marked one in the wave
+ * after its latest source's, with its initial guess fitted to its sources'
+ * solutions by the Galerkin condition x_a^T (b - K x0) = 0 on its own
+ * operator (least squares on the residual if that is worse than x0 = 0,
+ * and x0 = 0 if both are); an x0 given for a marked system is replaced.
+ * Sources may be marked themselves (a sweep of several levels), but a mark
+ * that would make a system depend on itself is refused.  nfrom = 0
+ * unmarks.  A mixed-precision batch solve (APHID_F_MIXED) ignores the
+ * marks. */
 #define APHID_SWEEP_MAX_FROM 4
 int aphid_batch_set_sweep_guess(aphid_batch *batch, int sys, int nfrom,
                                 const int *from);

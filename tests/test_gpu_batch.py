@@ -348,15 +348,18 @@ def _sweep_family(nf=9, seed=61):
     return base, np.logspace(-1, 3, nf), srcs
 
 
-@pytest.mark.parametrize("large", [False, True])
-def test_sweep_guess_from_neighbouring_frequencies(gpu_mod, oracle, monkeypatch, large):
+@pytest.mark.parametrize("large,levels", [(False, 2), (True, 2), (False, 3)])
+def test_sweep_guess_from_neighbouring_frequencies(gpu_mod, oracle, monkeypatch, large, levels):
     """Two-wave sweep: even frequencies from x0 = 0, then each odd one from
     its neighbours' solutions (same source), amplitudes fitted on its own
     operator.  Every system converges (true rel-res 1e-10) to the oracle's
-    fields (1e-7); each second-wave guess's reported ||b - K x0|| / ||b||
+    fields (1e-7); each fitted guess's reported ||b - K x0|| / ||b||
     equals the host's Galerkin fit of the same candidates (1e-6) and is
-    below 1; the second wave takes fewer iterations than from zero.  With
-    `large` every system takes the single-system path inside the batch."""
+    below 1; the fitted systems take fewer iterations than from zero.  With
+    `large` every system takes the single-system path inside the batch.
+    With three levels, every fourth frequency starts from zero, the ones
+    between them from those (second wave), and the odd ones from their
+    neighbours, some of which the second wave solved."""
     if large:
         monkeypatch.setenv("APHID_BATCH_SEQ_MIN", "1")
     base, freqs, srcs = _sweep_family()
@@ -379,9 +382,12 @@ def test_sweep_guess_from_neighbouring_frequencies(gpu_mod, oracle, monkeypatch,
     second = {}
     with gpu_mod.BatchSolver() as B:
         build(B)
-        for k in range(1, len(freqs), 2):
+        for k in range(len(freqs)):
+            if k % (1 << (levels - 1)) == 0:
+                continue
+            d = k & -k  # from the solutions d frequencies below and above
             for s in range(ns):
-                nb = [k - 1] + ([k + 1] if k + 1 < len(freqs) else [])
+                nb = [k - d] + ([k + d] if k + d < len(freqs) else [])
                 second[k * ns + s] = [kk * ns + s for kk in nb]
                 B.set_sweep_guess(k * ns + s, second[k * ns + s])
         reps, secs = B.solve(tol, 20000)
@@ -447,12 +453,15 @@ def test_sweep_guess_arguments(gpu_mod):
         with pytest.raises(gpu_mod.AphidError, match="source system"):
             B.set_sweep_guess(1, [1])
         B.set_sweep_guess(1, [0, 2])
-        with pytest.raises(gpu_mod.AphidError, match="first wave"):
+        with pytest.raises(gpu_mod.AphidError, match="cycle"):
             B.set_sweep_guess(2, [1])
-        with pytest.raises(gpu_mod.AphidError, match="source of another"):
-            B.set_sweep_guess(0, [2])
+        B.set_sweep_guess(0, [2])  # a third wave: 2, then 0, then 1
+        with pytest.raises(gpu_mod.AphidError, match="cycle"):
+            B.set_sweep_guess(2, [0])
         with pytest.raises(gpu_mod.AphidError, match="nfrom"):
             B.set_sweep_guess(1, [0, 2, 0, 2, 0])
+        B.set_sweep_guess(0, [])
+        B.set_sweep_guess(2, [0])  # allowed once 0 has no sources
 
 
 def single_rhs_system(gpu_mod, B, op, case):
