@@ -16,8 +16,10 @@
 ! residual norm, seconds, as each of the reference's processes writes   !
 ! it, src/aphid_qmr.f90:381; at most 32M / NSYS lines per system) and   !
 ! aphid_solution.bin (raw A_x, A_y, A_z, Phi).  With -s the batch       !
-! runs as a frequency sweep: per source, every second frequency in     !
-! ascending order starts from a fit of its two neighbours' solutions   !
+! runs as a frequency sweep of three levels: per source, in ascending  !
+! order (place q = 0, 1, ...), every fourth frequency starts from      !
+! x0 = 0 and each other one from a fit of the solutions d places below !
+! and above it, d the lowest set bit of q                              !
 ! (aphid_batch_set_sweep_guess).                                       !
 !----------------------------------------------------------------------!
       program aphid_gpu_parallel
@@ -47,7 +49,7 @@
       type(c_ptr) :: ctx, batch, pa, pb, pc, pd
       logical :: sweep
       integer, allocatable :: order(:)
-      integer :: g, nfrom
+      integer :: g, nfrom, qz, sd
       integer(c_int) :: from(2)
       real(c_double), allocatable :: hist(:), htime(:)
       integer(c_int) :: hcap, hlen
@@ -217,8 +219,10 @@
       end do
       rc = aphid_ctx_destroy(ctx)
 
-!     frequency sweep: per source, the frequencies in ascending order; each
-!     second one starts from its neighbours' solutions (same source)
+!     frequency sweep: per source, the frequencies in ascending order
+!     (place qz = f - 1); every fourth from x0 = 0, each other one from
+!     the solutions sd places below and above it (same source), sd the
+!     lowest set bit of qz
       if (sweep .and. nfreq >= 2) then
         allocate(order(nfreq))
         do f = 1, nfreq
@@ -235,12 +239,15 @@
           order(g + 1) = q
         end do
         do s = 1, nsys / nfreq
-          do f = 2, nfreq, 2
+          do f = 2, nfreq
+            qz = f - 1
+            if (mod(qz, 4) == 0) cycle
+            sd = iand(qz, -qz)
             nfrom = 1
-            from(1) = int((s - 1) * nfreq + order(f - 1) - 1, c_int)
-            if (f < nfreq) then
+            from(1) = int((s - 1) * nfreq + order(f - sd) - 1, c_int)
+            if (f + sd <= nfreq) then
               nfrom = 2
-              from(2) = int((s - 1) * nfreq + order(f + 1) - 1, c_int)
+              from(2) = int((s - 1) * nfreq + order(f + sd) - 1, c_int)
             end if
             rc = aphid_batch_set_sweep_guess(batch,                       &
                    int((s - 1) * nfreq + order(f) - 1, c_int),            &

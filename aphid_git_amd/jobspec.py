@@ -143,16 +143,25 @@ def tmp_dir_name(p: int, nprocs: int) -> str:
     return f"tmp-{p + 1:0{digits}d}"
 
 
-def sweep_guesses(freqs: Sequence[float], n_sources: int) -> Dict[int, List[int]]:
+def sweep_guesses(freqs: Sequence[float], n_sources: int, levels: int = 3) -> Dict[int, List[int]]:
     """The frequency-sweep schedule of a job: per source, the frequencies in
-    ascending order; every second one (system p = s*NFREQ + f) starts from
-    its neighbours' solutions.  Returns {second-wave system: source systems}."""
+    ascending order, q = 0, 1, ...; every 2^(levels-1)-th one starts from
+    x0 = 0, and each other one (system p = s*NFREQ + f) from the solutions
+    d places below and above it, d the lowest set bit of q (the batch
+    solves it in the wave after theirs).  levels = 2: every second
+    frequency from its two neighbours.  Returns {fitted system: source
+    systems}."""
+    if levels < 2:
+        raise ValueError("a sweep has at least 2 levels")
     nf = len(freqs)
     order = sorted(range(nf), key=lambda f: freqs[f])
     out = {}
     for s in range(n_sources):
-        for q in range(1, nf, 2):
-            nb = [order[q - 1]] + ([order[q + 1]] if q + 1 < nf else [])
+        for q in range(1, nf):
+            if q % (1 << (levels - 1)) == 0:
+                continue
+            d = q & -q
+            nb = [order[q - d]] + ([order[q + d]] if q + d < nf else [])
             out[s * nf + order[q]] = [s * nf + f for f in nb]
     return out
 
@@ -164,8 +173,9 @@ HISTORY_ENTRIES = 1 << 25
 def run_job(d: str, device: int = 0, out_dir: Optional[str] = None, write: bool = True,
             flags: int = 0, sweep: bool = False):
     """Solve every (source, frequency) of the job in d as one batched solve
-    (sweep: every second frequency of each source, in ascending order,
-    starts from a fit of its neighbours' solutions).  Returns (per-system
+    (sweep: the frequencies of each source, in ascending order, are solved
+    in the waves of sweep_guesses, all but every fourth from a fit of
+    solutions at neighbouring frequencies).  Returns (per-system
     reports, device seconds, wall seconds of the solve)."""
     job = read_job(d)
     out_dir = out_dir or d

@@ -56,10 +56,11 @@ def test_job_systems_match_the_oracle(tmp_path, oracle, stype, sweep):
     CPU oracle's complex128 QMR on the same (mode, frequency, source): the
     MT plane-wave modes 2 and 3 (src/aphid.f90:588-636) and CS dipole arrays
     (mode 5), 12^3 cells, rel-res 1e-10 on both sides.  Fields within 1e-7
-    (relative 2-norm over all unknowns).  With `sweep` every second
-    frequency of a source (in ascending order) starts from its neighbours'
-    solutions: its initial guess is better than x0 = 0 and its fields meet
-    the same bar.  Each tmp-N/qmr.iter holds one line per iteration
+    (relative 2-norm over all unknowns).  With `sweep` the frequencies of
+    a source (in ascending order) are solved in the waves of
+    jobspec.sweep_guesses, each fitted one from solutions at neighbouring
+    frequencies: its initial guess is better than x0 = 0 and its fields
+    meet the same bar.  Each tmp-N/qmr.iter holds one line per iteration
     (src/aphid_qmr.f90:381); from x0 = 0 its first 20 residual norms equal
     the oracle's history to 1e-6 and its length is within 10% of the
     oracle's iteration count."""

@@ -71,13 +71,20 @@ def test_process_directory_names(p, n, name):
 
 
 def test_sweep_schedule_pairs_neighbouring_frequencies():
-    """Per source, the frequencies in ascending order; every second one
-    starts from its neighbours (process p = s*NFREQ + f, src/aphid_parallel.f90
-    process order), also when the job lists its frequencies unsorted."""
+    """Per source, the frequencies in ascending order (process p =
+    s*NFREQ + f, src/aphid_parallel.f90 process order), also when the job
+    lists its frequencies unsorted.  Two levels: every second one starts
+    from its neighbours.  Three (the default): every fourth from zero, the
+    ones halfway between from those two places away, the rest from their
+    neighbours."""
     from aphid_git_amd import jobspec
 
     freqs = [30.0, 0.3, 3000.0, 3.0, 300.0]  # ascending: 1 3 0 4 2
-    g = jobspec.sweep_guesses(freqs, 2)
+    g = jobspec.sweep_guesses(freqs, 2, levels=2)
     assert g == {3: [1, 0], 4: [0, 2], 8: [6, 5], 9: [5, 7]}
+    g = jobspec.sweep_guesses(freqs, 2)
+    assert g == {3: [1, 0], 0: [1, 2], 4: [0, 2], 8: [6, 5], 5: [6, 7], 9: [5, 7]}
     assert jobspec.sweep_guesses([5.0], 3) == {}
     assert jobspec.sweep_guesses([2.0, 1.0], 1) == {0: [1]}
+    with pytest.raises(ValueError):
+        jobspec.sweep_guesses(freqs, 1, levels=1)
